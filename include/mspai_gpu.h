@@ -1,0 +1,121 @@
+/*
+ * mspai_gpu: MI355X (gfx950) builder for SPAI/MSPAI preconditioners.
+ *
+ * C ABI of the drop-in for the reference's per-column least-squares path.
+ * The reference (dmancy/mspai) runs that path as
+ *   Spai<T>::SPAI_Algorithm(A, M, B, P, UP, ep, mn, ns, hs, um, pk, pm, count, vb)
+ *     src/Spai.h:89-153, src/Spai.imp:33-191, called at src/imspai.cpp:831-833
+ * which loops Spai_Unrestrained<T>::SPAI_Column (src/Spai_Unrestrained.imp:33-133)
+ * over every column.  Here the whole loop is one device build over a
+ * GPU-resident CSC copy of A.  All functions return an mspai_status (0 = OK);
+ * nothing throws across this boundary.  A context is not thread safe.
+ */
+#ifndef MSPAI_GPU_H
+#define MSPAI_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mspai_ctx mspai_ctx;
+
+/* Error codes.  The reference throws std::runtime_error for bad options
+ * (src/imspai.cpp:239-480) and ignores LAPACK info (src/Spai_Sub.cpp:89-121);
+ * both become codes here. */
+typedef enum {
+    MSPAI_OK = 0,
+    MSPAI_ERR_ARG = -1,        /* bad argument or option (message: mspai_last_error) */
+    MSPAI_ERR_STATE = -2,      /* call order (e.g. build before upload) */
+    MSPAI_ERR_HIP = -3,        /* HIP runtime failure */
+    MSPAI_ERR_NO_DEVICE = -4,  /* no gfx950 device visible */
+    MSPAI_ERR_CAPACITY = -5,   /* a column exceeds the library's size limits */
+    MSPAI_ERR_UNSUPPORTED = -6 /* option combination the reference rejects */
+} mspai_status;
+
+/* Per-column status bits (mspai_download_column_info). */
+enum {
+    MSPAI_COL_SINGULAR = 1,    /* exact zero pivot: Q^T b left unsolved, as dtrtrs does */
+    MSPAI_COL_UNDERDET = 2,    /* |I| < |J| in some solve (solution set to 0) */
+    MSPAI_COL_NAN_RHO = 4,     /* mean of the candidate scores is NaN: nothing added */
+    MSPAI_COL_FALLBACK = 8,    /* mixed path: recomputed in FP64 (selection margin guard) */
+    MSPAI_COL_K_NOT_IN_I = 16  /* target column has no entry in I (src/Spai_Sub.imp:629-658) */
+};
+
+/* Options: the mspai keys of Command_Line (defaults src/Command_Line.cpp:192-226,
+ * PETSc keys src/imspai.cpp:87-218) that act on this path, plus GPU keys. */
+typedef struct {
+    double epsilon;        /* -ep  residual bound per column (default 0.4) */
+    int maxnew;            /* -mn  max new indices per augmentation step (5) */
+    int max_impr_steps;    /* -ns  augmentation steps (5); 0 = static SPAI */
+    int use_mean;          /* -um  1: accept rho <= mean; 0: threshold = sum (1) */
+    int qr;                /* -qr  0 = fresh QR per step, 2 = dense QR update (0) */
+    int cache;             /* -cs  device pattern->R cache entries, 0 = off (0) */
+    int precision;         /* GPU: 0 = FP64, 1 = FP32 QR + FP64 refinement (0) */
+    int binning;           /* GPU: 1 = size-binned kernels, 0 = one generic bin (1) */
+    int verbose;           /* -vb */
+} mspai_opts;
+
+/* Fill `o` with the reference's defaults. */
+void mspai_default_opts(mspai_opts* o);
+
+/* Timing and counters of the last build. */
+typedef struct {
+    double build_ms;        /* device time of the whole build (HIP events) */
+    double solve_ms;        /* device time of the column kernels only */
+    long long nnz;          /* nnz of M */
+    int ncols;
+    int n_fallback;         /* mixed path: columns recomputed in FP64 */
+    int n_overflow;         /* columns moved to a larger bin */
+    double work[8];         /* summed algorithmic counters: QR flop, other flop,
+                               gather B, candidate-union B, rho B, candidates,
+                               sum |I|, sum |J| (over all solves) */
+} mspai_stats;
+
+/* Context on HIP device `device` (replaces the MPI world of Com_Server). */
+int mspai_ctx_create(int device, mspai_ctx** ctx);
+int mspai_ctx_destroy(mspai_ctx* ctx);
+const char* mspai_last_error(const mspai_ctx* ctx);
+
+/*
+ * Upload C (nrows x ncols CSC, row indices sorted per column) and its row
+ * pattern (CSR: rowptr[nrows+1], colind sorted per row; NULL = derived on
+ * the host).  Replaces Compressed_Lines (src/Compressed_Lines.h:52-101):
+ * col_idcs/A/len_cols become rowind/val/colptr, row_idcs/len_rows become
+ * colind/rowptr.  nrows > ncols holds probing rows (src/Read_mm_Matrix.cpp:532-667).
+ */
+int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int* colptr, const int* rowind,
+                     const double* val, const int* rowptr, const int* colind);
+
+/* Replace only the values of the uploaded matrix (same pattern). */
+int mspai_update_values(mspai_ctx* ctx, const double* val);
+
+/* Start pattern J0 per column (Pattern::j_sets, src/Pattern.h:69-120,
+ * Com_Server::Get_P_Col src/Com_Server.imp:445-470).  NULL pointers select
+ * the diagonal pattern (-pp 1, src/Pattern.cpp:378-423). */
+int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* rowind);
+
+/* Target B (nrows x ncols CSC) for MSPAI probing (Create_bk_Hat,
+ * src/Spai_Sub.imp:735-775); colptr == NULL restores the identity. */
+int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* rowind, const double* val);
+
+/* Build M (Spai<T>::SPAI_Algorithm, src/Spai.imp:33-191).  M stays on the
+ * device until downloaded.  `stats` may be NULL. */
+int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats);
+
+/* nnz of the last built M. */
+int mspai_result_nnz(const mspai_ctx* ctx, long long* nnz);
+
+/* Copy M out as CSC: colptr[ncols+1], rowind[nnz] (sorted J per column),
+ * val[nnz] (Insert_Row_Solution, src/Com_Server.cpp:415-457). */
+int mspai_download_csc(mspai_ctx* ctx, int* colptr, int* rowind, double* val);
+
+/* Per-column results: final residual norm ||A(I,J) m - b(I)||, number of
+ * least-squares solves, status bits, and 8 work counters per column.
+ * Any pointer may be NULL. */
+int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int* nsolve, int* status,
+                               double* work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

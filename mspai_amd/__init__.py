@@ -1,0 +1,183 @@
+"""mspai_amd: MI355X builder for SPAI/MSPAI preconditioners.
+
+Python front end of the C ABI in include/mspai_gpu.h (libmspai_gpu.so).  The
+library is the product; this module only moves arrays across the boundary.
+There is no CPU fallback: if the library is missing or no gfx950 device is
+visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmspai_gpu.so")
+NWORK = 8
+
+COL_SINGULAR, COL_UNDERDET, COL_NAN_RHO, COL_FALLBACK, COL_K_NOT_IN_I = 1, 2, 4, 8, 16
+
+EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspai_last_error",
+            "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_set_target",
+            "mspai_build", "mspai_result_nnz", "mspai_download_csc",
+            "mspai_download_column_info"]
+
+
+class Opts(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("maxnew", C.c_int), ("max_impr_steps", C.c_int),
+                ("use_mean", C.c_int), ("qr", C.c_int), ("cache", C.c_int),
+                ("precision", C.c_int), ("binning", C.c_int), ("verbose", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("solve_ms", C.c_double), ("nnz", C.c_longlong),
+                ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int),
+                ("work", C.c_double * NWORK)]
+
+
+_lib = None
+
+
+def load():
+    """Load libmspai_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                           "(or python mspai_amd/build.py) first")
+    lib = C.CDLL(LIB_PATH)
+    for name in EXPORTED:
+        getattr(lib, name)
+    lib.mspai_last_error.restype = C.c_char_p
+    lib.mspai_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class MspaiError(RuntimeError):
+    pass
+
+
+class Builder:
+    """One device context: upload C, set pattern/target, build, download M."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mspai_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise MspaiError(f"mspai_ctx_create failed ({rc}): no gfx950 device / HIP error")
+        self.h = h
+        self.shape = None
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.lib.mspai_last_error(self.h)
+            raise MspaiError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mspai_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, A: sp.csc_matrix, row_pattern=None):
+        A = sp.csc_matrix(A)
+        A.sort_indices()
+        ip, ix, v = _i32(A.indptr), _i32(A.indices), _f64(A.data)
+        rp = ri = None
+        if row_pattern is not None:
+            rp, ri = _i32(row_pattern[0]), _i32(row_pattern[1])
+        self._chk(self.lib.mspai_upload_csc(self.h, A.shape[0], A.shape[1], _p(ip), _p(ix), _p(v),
+                                            _p(rp), _p(ri)), "mspai_upload_csc")
+        self.shape = A.shape
+
+    def update_values(self, vals):
+        v = _f64(vals)
+        self._chk(self.lib.mspai_update_values(self.h, _p(v)), "mspai_update_values")
+
+    def set_pattern(self, P=None):
+        if P is None:
+            self._chk(self.lib.mspai_set_pattern(self.h, None, None), "mspai_set_pattern")
+            return
+        P = sp.csc_matrix(P)
+        P.sort_indices()
+        self._chk(self.lib.mspai_set_pattern(self.h, _p(_i32(P.indptr)), _p(_i32(P.indices))),
+                  "mspai_set_pattern")
+
+    def set_target(self, B=None):
+        if B is None:
+            self._chk(self.lib.mspai_set_target(self.h, None, None, None), "mspai_set_target")
+            return
+        B = sp.csc_matrix(B)
+        B.sort_indices()
+        self._chk(self.lib.mspai_set_target(self.h, _p(_i32(B.indptr)), _p(_i32(B.indices)),
+                                            _p(_f64(B.data))), "mspai_set_target")
+
+    def build(self, epsilon=0.4, maxnew=5, max_impr_steps=5, use_mean=1, qr=0, cache=0,
+              precision=0, binning=1, verbose=0):
+        o = Opts(epsilon, maxnew, max_impr_steps, use_mean, qr, cache, precision, binning, verbose)
+        st = Stats()
+        self._chk(self.lib.mspai_build(self.h, C.byref(o), C.byref(st)), "mspai_build")
+        return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz,
+                "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow,
+                "work": np.array(st.work[:])}
+
+    def download(self) -> sp.csc_matrix:
+        nnz = C.c_longlong()
+        self._chk(self.lib.mspai_result_nnz(self.h, C.byref(nnz)), "mspai_result_nnz")
+        n = self.shape[1]
+        ptr = np.zeros(n + 1, np.int32)
+        ind = np.zeros(max(1, nnz.value), np.int32)
+        val = np.zeros(max(1, nnz.value), np.float64)
+        self._chk(self.lib.mspai_download_csc(self.h, _p(ptr), _p(ind), _p(val)), "mspai_download_csc")
+        return sp.csc_matrix((val[:nnz.value], ind[:nnz.value], ptr), shape=(n, n))
+
+    def column_info(self):
+        n = self.shape[1]
+        res = np.zeros(n, np.float64)
+        ns = np.zeros(n, np.int32)
+        stt = np.zeros(n, np.int32)
+        wk = np.zeros(n * NWORK, np.float64)
+        self._chk(self.lib.mspai_download_column_info(self.h, _p(res), _p(ns), _p(stt), _p(wk)),
+                  "mspai_download_column_info")
+        return {"res_norm": res, "nsolve": ns, "status": stt, "work": wk.reshape(n, NWORK)}
+
+
+def spai(prob, device: int = 0, **opts):
+    """Build M for a mspai_amd.gen.Problem on the GPU.
+    Returns (M as scipy CSC, per-column info, build stats)."""
+    b = Builder(device)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        b.set_target(prob.B)
+        kw = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                  use_mean=prob.use_mean)
+        kw.update(opts)
+        stats = b.build(**kw)
+        M = b.download()
+        info = b.column_info()
+    finally:
+        b.close()
+    return M, info, stats

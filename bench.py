@@ -1,0 +1,196 @@
+"""Benchmark: SPAI columns/s on one MI355X.
+
+Workload (the configuration the metric is quoted on): adaptive SPAI on the 2D
+upwind convection-diffusion operator, 256 x 256 grid (n = 65,536), eps 0.4,
+5 augmentation steps, 5 new indices per step, start pattern diag(A).  One
+step = one complete build of M (every column of M) on a device-resident
+copy of A; inputs are uploaded before the timed region.
+
+Prints one JSON line.  `value` is the columns/s of the FP64 build.
+`roofline` refers to the column kernel (HIP events on the library's stream);
+`cpu_baseline` times the CPU FP64 restatement of the reference on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _algorithmic_bytes(work, nnzM, n):
+    """Gather + candidate union + rho reads + the CSC output of M."""
+    return float(work[2] + work[3] + work[4] + 12.0 * nnzM + 4.0 * (n + 1))
+
+
+def _traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{name}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def _fro_residual(A, M):
+    R = (A @ M).tocsc()
+    R = R - __import__("scipy.sparse", fromlist=["identity"]).identity(A.shape[1], format="csc")
+    return float(np.sqrt((R.multiply(R)).sum()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
+                    help="timed paths; with both, value is the mixed path")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (scale for rocprofv3 FETCH/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
+        return
+    if args.gpus != 1:
+        print("bench.py builds on one GPU only: the column solve shards across GPUs by "
+              "columns (see DESIGN.md), but multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mspai_amd
+    from mspai_amd import gen
+
+    prob = gen.config2(args.nx)
+    n = prob.n
+    b = mspai_amd.Builder(0)
+    b.upload(prob.A, prob.row_pattern())
+    b.set_pattern(prob.P)
+    base_opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                     use_mean=prob.use_mean)
+
+    def timed(precision):
+        opts = dict(base_opts, precision=precision)
+        for _ in range(args.warmup):
+            b.build(**opts)
+        builds, solves = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = b.build(**opts)
+            builds.append(st["build_ms"])
+            solves.append(st["solve_ms"])
+        wall = time.perf_counter() - t0
+        return {"ms_per_step": 1000.0 * wall / args.steps, "dev_ms": float(np.mean(builds)),
+                "kern_ms": float(np.mean(solves)), "M": b.download(), "work": b.work_totals(),
+                "bins": b.bin_counts(), "n_fallback": st["n_fallback"], "n_overflow": st["n_overflow"]}
+
+    r64 = timed(0)
+    rmx = timed(1) if args.precision in ("mixed", "both") else None
+    b.close()
+    main_run = rmx if rmx is not None else r64
+    M = main_run["M"]
+    ms_per_step = main_run["ms_per_step"]
+    work = main_run["work"]
+    kern_ms = main_run["kern_ms"]
+    alg_bytes = _algorithmic_bytes(work, M.nnz, n)
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = _traffic("convdiff2d_256")
+    fro_gpu = _fro_residual(prob.A, M)
+    M64 = r64["M"]
+    mixed_check = None
+    if rmx is not None:
+        same_pat = bool(np.array_equal(M64.indptr, M.indptr) and np.array_equal(M64.indices, M.indices))
+        rel = float(np.max(np.abs(M.data - M64.data) / np.maximum(np.abs(M64.data), 1e-300))) if same_pat else None
+        mixed_check = {"pattern_equal_fp64": same_pat, "max_rel_err_vs_fp64": rel,
+                       "fro_AM_minus_I_fp64": _fro_residual(prob.A, M64),
+                       "n_fp64_fallback": rmx["n_fallback"]}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = os.cpu_count() or 1
+        cores = min(cores, 16)
+        t = time.perf_counter()
+        reps = 0
+        while True:
+            ref = O.spai(prob, threads=cores)
+            reps += 1
+            if time.perf_counter() - t > 5.0 or reps >= 50:
+                break
+        cpu_t = (time.perf_counter() - t) / reps
+        t = time.perf_counter()
+        O.spai(prob, threads=1)
+        cpu1 = time.perf_counter() - t
+        Mref = O.to_csc(ref, n)
+        parity = {"fp64_pattern_equal": bool(np.array_equal(Mref.indptr, M64.indptr)
+                                             and np.array_equal(Mref.indices, M64.indices)),
+                  "fp64_values_bitwise": bool(np.array_equal(Mref.data.view(np.int64), M64.data.view(np.int64))),
+                  "reported_pattern_equal": bool(np.array_equal(Mref.indptr, M.indptr)
+                                                 and np.array_equal(Mref.indices, M.indices)),
+                  "fro_AM_minus_I_oracle": _fro_residual(prob.A, Mref)}
+        cpu = {"value": n / cpu_t, "unit": "columns/s", "cores": cores, "kind": "port",
+               "sample": f"full workload (n={n}) x {reps} runs on {cores} threads; "
+                         f"1 thread: {n / cpu1:.0f} columns/s",
+               "single_thread_value": n / cpu1}
+
+    value = n / (ms_per_step * 1e-3)
+    out = {
+        "metric": "SPAI columns/s, mixed vs FP64 GPU at matched ||AM-I||_F; bwd/fwd time",
+        "value": value,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 QR + f64 refinement" if rmx is not None else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"adaptive SPAI, 2D upwind convection-diffusion {args.nx}x{args.nx} "
+                               f"(n={n}), eps=0.4, ns=5, mn=5, J0=diag",
+                   "n_columns": n, "nnz_A": int(prob.A.nnz), "nnz_M": int(M.nnz),
+                   "precision": "mixed" if rmx is not None else "fp64",
+                   "parallelism": "single GPU, persistent column queue"},
+        "device_ms_per_build": main_run["dev_ms"],
+        "fp64_columns_per_s": n / (r64["ms_per_step"] * 1e-3),
+        "fp64_device_ms_per_build": r64["dev_ms"],
+        "mixed_columns_per_s": (n / (rmx["ms_per_step"] * 1e-3)) if rmx is not None else None,
+        "mixed_over_fp64": (r64["ms_per_step"] / rmx["ms_per_step"]) if rmx is not None else None,
+        "mixed_check": mixed_check,
+        "fro_AM_minus_I": fro_gpu,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "spai_column_kernel (all size bins)", "kernel_ms": kern_ms,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "qr_flop_per_launch": float(work[0] + work[1])},
+        "bins": main_run["bins"],
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

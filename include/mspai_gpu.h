@@ -65,10 +65,7 @@ typedef struct {
     long long nnz;          /* nnz of M */
     int ncols;
     int n_fallback;         /* mixed path: columns recomputed in FP64 */
-    int n_overflow;         /* columns moved to a larger bin */
-    double work[8];         /* summed algorithmic counters: QR flop, other flop,
-                               gather B, candidate-union B, rho B, candidates,
-                               sum |I|, sum |J| (over all solves) */
+    int n_overflow;         /* columns that outgrew their size bin and moved on */
 } mspai_stats;
 
 /* Context on HIP device `device` (replaces the MPI world of Com_Server). */
@@ -114,6 +111,18 @@ int mspai_download_csc(mspai_ctx* ctx, int* colptr, int* rowind, double* val);
  * Any pointer may be NULL. */
 int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int* nsolve, int* status,
                                double* work);
+
+/* Sums over all columns of the 8 work counters of the last build:
+ * QR flop, other solve flop (Q^T b, triangular solve, residual), gather
+ * bytes, candidate-union bytes, rho bytes, candidates scored, sum |I|,
+ * sum |J| (over all least-squares solves). */
+int mspai_work_totals(mspai_ctx* ctx, double* totals);
+
+/* Size-bin routing of the last build: nbins, and for b = 0..nbins the
+ * columns routed to bin b up front and the columns it processed (routed plus
+ * overflow from bin b-1); bin nbins is the generic kernel.  Arrays hold
+ * nbins + 1 entries (at most 8). */
+int mspai_bin_counts(const mspai_ctx* ctx, int* nbins, int* routed, int* queued);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ COL_SINGULAR, COL_UNDERDET, COL_NAN_RHO, COL_FALLBACK, COL_K_NOT_IN_I = 1, 2, 4,
 EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspai_last_error",
             "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_set_target",
             "mspai_build", "mspai_result_nnz", "mspai_download_csc",
-            "mspai_download_column_info"]
+            "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts"]
 
 
 class Opts(C.Structure):
@@ -33,8 +33,7 @@ class Opts(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("solve_ms", C.c_double), ("nnz", C.c_longlong),
-                ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int),
-                ("work", C.c_double * NWORK)]
+                ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int)]
 
 
 _lib = None
@@ -140,8 +139,20 @@ class Builder:
         st = Stats()
         self._chk(self.lib.mspai_build(self.h, C.byref(o), C.byref(st)), "mspai_build")
         return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz,
-                "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow,
-                "work": np.array(st.work[:])}
+                "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow}
+
+    def work_totals(self):
+        w = np.zeros(NWORK, np.float64)
+        self._chk(self.lib.mspai_work_totals(self.h, _p(w)), "mspai_work_totals")
+        return w
+
+    def bin_counts(self):
+        nb = C.c_int()
+        routed = np.zeros(8, np.int32)
+        queued = np.zeros(8, np.int32)
+        self._chk(self.lib.mspai_bin_counts(self.h, C.byref(nb), _p(routed), _p(queued)), "mspai_bin_counts")
+        return {"nbins": nb.value, "routed": routed[:nb.value + 1].tolist(),
+                "queued": queued[:nb.value + 1].tolist()}
 
     def download(self) -> sp.csc_matrix:
         nnz = C.c_longlong()
@@ -178,6 +189,8 @@ def spai(prob, device: int = 0, **opts):
         stats = b.build(**kw)
         M = b.download()
         info = b.column_info()
+        stats["work"] = b.work_totals()
+        stats["bins"] = b.bin_counts()
     finally:
         b.close()
     return M, info, stats

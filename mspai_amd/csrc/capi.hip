@@ -24,6 +24,8 @@
 
 using namespace mspai;
 
+constexpr int kMaxBins = 4;
+
 namespace {
 
 template <class T>
@@ -45,11 +47,22 @@ __global__ void column_norms_kernel(DevCsc A, double* ajn)
     ajn[j] = sqrt(s);
 }
 
-__global__ void iota_kernel(int* q, int n)
+/* Sum the per-column work counters (one block, fixed order). */
+__global__ void work_sum_kernel(const double* work, int n, double* out)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) q[i] = i;
+    __shared__ double part[256];
+    int q = blockIdx.x;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) s += work[(size_t)j * kNumWork + q];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = part[0];
 }
+
 
 __global__ void scatter_columns_kernel(int ncols, int cap, const int* m_len, const int* m_idx,
                                        const double* m_val, const int* colptr, int* rowind, double* val)
@@ -100,14 +113,18 @@ struct mspai_ctx {
     long long nnzM = -1;
     size_t Mcap = 0;
     // queues and scratch
-    int* d_queue = nullptr;
-    int* d_counter = nullptr;  // [0] queue head, [1] overflow count
-    int* d_overflow = nullptr;
+    int* d_queues = nullptr;   // (kMaxBins + 1) queues of ncols entries
+    int* d_counter = nullptr;  // FP64 queues: [0..8) lengths, [8..16) heads; [16..24) routed lengths;
+                               // mixed queues: [24..32) lengths, [32..40) heads
+    double* d_wsum = nullptr;  // work-counter totals
     char* d_scratch = nullptr;
     size_t scratch_bytes = 0;
     void* d_cub = nullptr;
     size_t cub_bytes = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int last_nbins = 0;
+    int last_routed[8] = {0};
+    int last_queued[8] = {0};
 };
 
 #define HIPCHK(x)                                                                              \
@@ -170,7 +187,8 @@ extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
             delete ctx;
             return MSPAI_ERR_HIP;
         }
-    if (hipMalloc((void**)&ctx->d_counter, 4 * sizeof(int)) != hipSuccess) {
+    if (hipMalloc((void**)&ctx->d_counter, 64 * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_wsum, kNumWork * sizeof(double)) != hipSuccess) {
         delete ctx;
         return MSPAI_ERR_HIP;
     }
@@ -204,9 +222,9 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_Mptr);
     dfree(ctx->d_Mind);
     dfree(ctx->d_Mval);
-    dfree(ctx->d_queue);
+    dfree(ctx->d_queues);
     dfree(ctx->d_counter);
-    dfree(ctx->d_overflow);
+    dfree(ctx->d_wsum);
     dfree(ctx->d_scratch);
     if (ctx->d_cub) (void)hipFree(ctx->d_cub);
     for (auto& e : ctx->ev)
@@ -385,8 +403,7 @@ static int ensure_outputs(mspai_ctx* ctx, int cap)
     dfree(ctx->d_Mptr);
     dfree(ctx->d_Mind);
     dfree(ctx->d_Mval);
-    dfree(ctx->d_queue);
-    dfree(ctx->d_overflow);
+    dfree(ctx->d_queues);
     HIPCHK(hipMalloc((void**)&ctx->d_m_len, n * sizeof(int)));
     HIPCHK(hipMalloc((void**)&ctx->d_m_idx, n * cap * sizeof(int)));
     HIPCHK(hipMalloc((void**)&ctx->d_m_val, n * cap * sizeof(double)));
@@ -397,14 +414,14 @@ static int ensure_outputs(mspai_ctx* ctx, int cap)
     HIPCHK(hipMalloc((void**)&ctx->d_Mptr, (n + 1) * sizeof(int)));
     HIPCHK(hipMalloc((void**)&ctx->d_Mind, n * cap * sizeof(int)));
     HIPCHK(hipMalloc((void**)&ctx->d_Mval, n * cap * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&ctx->d_queue, n * sizeof(int)));
-    HIPCHK(hipMalloc((void**)&ctx->d_overflow, n * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&ctx->d_queues, (2 * kMaxBins + 1) * n * sizeof(int)));
     ctx->cap = cap;
     ctx->out_cols = n;
     ctx->Mcap = n * cap;
     /* scan workspace for the compaction */
     size_t need = 0;
     hipcub::DeviceScan::InclusiveSum(nullptr, need, ctx->d_m_len, ctx->d_Mptr + 1, (int)n, ctx->stream);
+    need = std::max<size_t>(need, kNumWork * sizeof(double));
     if (need > ctx->cub_bytes) {
         if (ctx->d_cub) (void)hipFree(ctx->d_cub);
         ctx->d_cub = nullptr;
@@ -423,44 +440,99 @@ static size_t next_pow2(size_t x)
     return p;
 }
 
-/* Size the generic kernel's per-wave scratch from worst-case bounds:
- * |J| <= cap, |I| <= min(nrows, cap * max nnz per column),
- * raw candidate list <= (|I| + 1) * max nnz per row. */
-static ScratchLayout generic_layout(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive)
+/* Byte offsets of one per-wave working set holding the given sizes. */
+static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capSort, int maxnew, size_t capPairs)
 {
     ScratchLayout L{};
-    size_t rawI = (size_t)cap * std::max(1, ctx->maxcolnnz);
-    size_t capI = std::min<size_t>(ctx->nrows, rawI);
-    capI = std::max<size_t>(capI, cap);
-    size_t rawU = adaptive ? (capI + 1) * std::max(1, ctx->maxrownnz) : 1;
-    size_t capU = adaptive ? std::min<size_t>(ctx->ncols, rawU) : 1;
-    size_t capSort = next_pow2(std::max(rawI, rawU));
-    L.capJ = cap;
+    capI = std::max<size_t>(capI, 1);
+    capU = std::max<size_t>(capU, 1);
+    capSort = next_pow2(std::max<size_t>(capSort, 1));
+    L.capJ = capJ;
     L.capI = (int)capI;
     L.capU = (int)capU;
     L.capSort = (int)capSort;
+    L.capPairs = (int)std::max<size_t>(1, capPairs);
     size_t o = 0;
     auto take = [&](size_t bytes) {
         size_t at = o;
-        o = align_up(o + bytes, 256);
+        o = align_up(o + bytes, 16);
         return at;
     };
-    L.J = take((size_t)cap * 4 + 64);
+    size_t capB = std::max<size_t>(capI, (size_t)capJ);
+    L.J = take((size_t)capJ * 4 + 16);
     L.I = take(capI * 4);
     L.sbuf = take(capSort * 4);
-    L.ubuf = take(capU * 4);
-    L.ahat = take(capI * cap * 8);
-    L.aqr = take(capI * cap * 8);
-    L.b = take(capI * 8);
-    L.b0 = take(capI * 8);
-    L.r = take(capI * 8);
-    L.tau = take((size_t)cap * 8);
+    L.aqr = take(capI * capJ * 8);
+    L.b = take(capB * 8);
+    L.b0 = take(capB * 8);
+    L.r = take(capB * 8);
+    L.tau = take((size_t)capJ * 8);
     L.jt = take(capU * 4);
     L.rho = take(capU * 8);
-    L.pairs = take(capU * sizeof(RhoPair));
+    L.pairs = take((size_t)L.capPairs * sizeof(RhoPair));
     L.sel = take((size_t)std::max(1, maxnew) * 4);
-    L.per_wave = o;
+    L.stack = take(3 * introsort_detail::kStackFrames * sizeof(int));
+    L.bq32 = take(capB * 4);
+    L.g = take((size_t)capJ * 8);
+    L.y = take((size_t)capJ * 8);
+    L.per_wave = align_up(o, 256);
     return L;
+}
+
+/* Worst-case layout of the generic kernel: |J| <= cap,
+ * |I| <= min(nrows, cap * max nnz per column), raw candidate list
+ * <= (|I| + 1) * max nnz per row. */
+static ScratchLayout generic_layout(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive)
+{
+    size_t rawI = (size_t)cap * std::max(1, ctx->maxcolnnz);
+    size_t capI = std::max<size_t>(std::min<size_t>(ctx->nrows, rawI), 1);
+    size_t rawU = adaptive ? (capI + 1) * std::max(1, ctx->maxrownnz) : 1;
+    size_t capU = adaptive ? std::min<size_t>(ctx->ncols, rawU) : 1;
+    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU);
+}
+
+/* One size bin: a lane-group width and the LDS layout of one group. */
+struct Bin {
+    int group;
+    ScratchLayout L;
+};
+
+/* LDS layouts of the size bins, smallest first: columns that outgrow a bin
+ * move to the next one; what outgrows the last runs generic.  A bin that
+ * already holds every column (by the worst-case bounds) ends the list. */
+static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive,
+                                    const ScratchLayout& G)
+{
+    struct Edge {
+        int group, J, I, U, sort, pairs;
+    };
+    const Edge edges[4] = {{16, 8, 16, 64, 128, 64},
+                           {32, 16, 32, 128, 256, 64},
+                           {64, 32, 64, 256, 512, 128},
+                           {64, 40, 128, 512, 1024, 128}};
+    std::vector<Bin> out;
+    for (const Edge& e : edges) {
+        int capJ = std::min(cap, e.J);
+        size_t capI = std::min<size_t>(G.capI, e.I);
+        size_t capU = adaptive ? std::min<size_t>(G.capU, e.U) : 1;
+        size_t capSort = std::min<size_t>(G.capSort, e.sort);
+        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, std::min<size_t>(capU, e.pairs));
+        if (L.per_wave * (64 / e.group) > 64 * 1024) continue;
+        out.push_back({e.group, L});
+        if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU) &&
+            L.capPairs >= G.capPairs)
+            break;
+    }
+    return out;
+}
+
+static int ensure_scratch(mspai_ctx* ctx, size_t need)
+{
+    if (need <= ctx->scratch_bytes) return MSPAI_OK;
+    dfree(ctx->d_scratch);
+    HIPCHK(hipMalloc((void**)&ctx->d_scratch, need));
+    ctx->scratch_bytes = need;
+    return MSPAI_OK;
 }
 
 extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats)
@@ -479,24 +551,20 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     if (cap <= 0) return fail(ctx, MSPAI_ERR_ARG, "build: empty start pattern and no augmentation");
     if ((rc = ensure_outputs(ctx, cap))) return rc;
     const bool adaptive = ns > 0;
-    ScratchLayout L = generic_layout(ctx, cap, opts->maxnew, adaptive);
-    int nwaves = std::max(1, ctx->num_cu) * 8;
-    /* keep the scratch within 16 GiB; fewer resident waves if needed */
+    const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive);
+    std::vector<Bin> bins;
+    if (opts->binning) bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G);
+    /* the generic kernel: keep its HBM working sets within 16 GiB */
+    int gwaves = std::max(1, ctx->num_cu) * 4;
     const size_t budget = (size_t)16 << 30;
-    while (nwaves > ctx->num_cu && (size_t)nwaves * L.per_wave > budget) nwaves /= 2;
-    size_t need = (size_t)nwaves * L.per_wave;
-    if (need > budget) return fail(ctx, MSPAI_ERR_CAPACITY, "build: per-column working set too large");
-    if (need > ctx->scratch_bytes) {
-        dfree(ctx->d_scratch);
-        HIPCHK(hipMalloc((void**)&ctx->d_scratch, need));
-        ctx->scratch_bytes = need;
-    }
+    while (gwaves > 1 && (size_t)gwaves * G.per_wave > budget) gwaves /= 2;
+    if ((size_t)gwaves * G.per_wave > budget) return fail(ctx, MSPAI_ERR_CAPACITY, "build: per-column working set too large");
+    if ((rc = ensure_scratch(ctx, (size_t)gwaves * G.per_wave))) return rc;
 
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     DevCsc A{ctx->nrows, ctx->ncols, ctx->d_colptr, ctx->d_rowind, ctx->d_val};
     hipLaunchKernelGGL(column_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A, ctx->d_ajn);
-    hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_queue, n);
-    HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 4 * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 64 * sizeof(int), ctx->stream));
 
     BuildArgs a{};
     a.A = A;
@@ -511,11 +579,6 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     a.ns = ns;
     a.use_mean = opts->use_mean;
     a.cap = cap;
-    a.queue = ctx->d_queue;
-    a.nqueue = n;
-    a.counter = ctx->d_counter;
-    a.overflow = ctx->d_overflow;
-    a.n_overflow = ctx->d_counter + 1;
     a.m_len = ctx->d_m_len;
     a.m_idx = ctx->d_m_idx;
     a.m_val = ctx->d_m_val;
@@ -524,10 +587,68 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     a.status = ctx->d_status;
     a.work = ctx->d_work;
     a.scratch = ctx->d_scratch;
-    a.L = L;
+    a.status_or = 0;
 
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    launch_generic_fp64(a, nwaves, ctx->stream);
+    /* Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb = generic),
+     * mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills the
+     * first queues; a bin's overflow appends to the next queue of its chain;
+     * the mixed chain ends in FP64 queue 0, which also takes the columns the
+     * mixed guards decline. */
+    const int nb = (int)bins.size();
+    const bool mixed = opts->precision == 1 && nb > 0;
+    BinEdges edges{};
+    for (int b = 0; b < nb; ++b) {
+        edges.capJ[b] = bins[b].L.capJ;
+        edges.capSort[b] = bins[b].L.capSort;
+    }
+    int* counts = ctx->d_counter;
+    int* heads = ctx->d_counter + 8;
+    int* mcounts = ctx->d_counter + 24;
+    int* mheads = ctx->d_counter + 32;
+    int* fq = ctx->d_queues;
+    int* mq = ctx->d_queues + (size_t)(kMaxBins + 1) * n;
+    if (mixed) {
+        /* routing into the mixed chain; "generic" (nb) lands in FP64 queue 0 */
+        launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->stream);
+        HIPCHK(hipGetLastError());
+        launch_copy_ints(mcounts, ctx->d_counter + 16, nb, ctx->stream);
+        launch_copy_ints(counts, ctx->d_counter + 16 + nb, 1, ctx->stream);
+        a.fallback = fq;
+        a.n_fallback = counts;
+        for (int b = 0; b < nb; ++b) {
+            const ScratchLayout& L = bins[b].L;
+            const int grp = bins[b].group;
+            a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
+            a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
+            size_t lds = L.per_wave * (64 / grp);
+            int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+            launch_binned(a, L, grp, true, mq + (size_t)b * n, mcounts + b, mheads + b, ctx->num_cu * per_cu,
+                          ctx->stream);
+            HIPCHK(hipGetLastError());
+        }
+        a.status_or = COL_FALLBACK;
+    } else {
+        launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->stream);
+        HIPCHK(hipGetLastError());
+        launch_copy_ints(counts, ctx->d_counter + 16, nb + 1, ctx->stream);
+    }
+    a.fallback = nullptr;
+    a.n_fallback = nullptr;
+    for (int b = 0; b < nb; ++b) {
+        const ScratchLayout& L = bins[b].L;
+        const int grp = bins[b].group;
+        a.overflow = fq + (size_t)(b + 1) * n;
+        a.n_overflow = counts + b + 1;
+        size_t lds = L.per_wave * (64 / grp);
+        int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+        launch_binned(a, L, grp, false, fq + (size_t)b * n, counts + b, heads + b, ctx->num_cu * per_cu,
+                      ctx->stream);
+        HIPCHK(hipGetLastError());
+    }
+    a.overflow = fq;  // never written: the generic layout holds every column
+    a.n_overflow = counts + 7;
+    launch_fp64_generic(a, G, fq + (size_t)nb * n, counts + nb, heads + nb, gwaves, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
 
@@ -554,13 +675,50 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         stats->solve_ms = t_solve;
         stats->nnz = last;
         stats->ncols = n;
-        std::vector<double> w((size_t)n * kNumWork);
-        HIPCHK(hipMemcpy(w.data(), ctx->d_work, w.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int q = 0; q < kNumWork; ++q) {
-            double s = 0.0;
-            for (int j = 0; j < n; ++j) s += w[(size_t)j * kNumWork + q];
-            stats->work[q] = s;
+        int cnt[64];
+        HIPCHK(hipMemcpy(cnt, ctx->d_counter, sizeof(cnt), hipMemcpyDeviceToHost));
+        const int nbb = (int)bins.size();
+        ctx->last_nbins = nbb;
+        if (mixed) {
+            /* FP64 queue 0 = routed-to-generic + last-bin overflow + guard declines */
+            int moved = 0;
+            for (int b = 1; b < nbb; ++b) moved += cnt[24 + b] - cnt[16 + b];
+            stats->n_overflow = moved;
+            stats->n_fallback = cnt[0] - cnt[16 + nbb];
+            for (int b = 0; b < 8; ++b) {
+                ctx->last_routed[b] = cnt[16 + b];
+                ctx->last_queued[b] = b < nbb ? cnt[24 + b] : (b == nbb ? cnt[0] : 0);
+            }
+        } else {
+            for (int b = 0; b <= nbb; ++b) stats->n_overflow += cnt[b] - cnt[16 + b];
+            for (int b = 0; b < 8; ++b) {
+                ctx->last_routed[b] = cnt[16 + b];
+                ctx->last_queued[b] = cnt[b];
+            }
         }
+    }
+    return MSPAI_OK;
+}
+extern "C" int mspai_work_totals(mspai_ctx* ctx, double* totals)
+{
+    if (!ctx || !totals) return MSPAI_ERR_ARG;
+    if (ctx->nnzM < 0) return fail(ctx, MSPAI_ERR_STATE, "work_totals: build first");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(work_sum_kernel, dim3(kNumWork), dim3(256), 0, ctx->stream, ctx->d_work, ctx->ncols,
+                       ctx->d_wsum);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(totals, ctx->d_wsum, kNumWork * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_bin_counts(const mspai_ctx* ctx, int* nbins, int* routed, int* queued)
+{
+    if (!ctx || !nbins) return MSPAI_ERR_ARG;
+    *nbins = ctx->last_nbins;
+    for (int b = 0; b <= ctx->last_nbins && b < 8; ++b) {
+        if (routed) routed[b] = ctx->last_routed[b];
+        if (queued) queued[b] = ctx->last_queued[b];
     }
     return MSPAI_OK;
 }

@@ -35,11 +35,22 @@ struct DevCsc {
     const double* val;
 };
 
-/* Per-wave scratch layout of the generic kernel (byte offsets). */
+/* Per-wave working-set layout (byte offsets) and the sizes it holds.  One
+ * layout per size bin; the generic layout is sized for the worst case. */
 struct ScratchLayout {
-    size_t J, I, sbuf, ubuf, ahat, aqr, b, b0, r, tau, jt, rho, pairs, sel;
+    size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y;
     size_t per_wave;
-    int capJ, capI, capU, capSort;
+    int capJ;      // |J|
+    int capI;      // |I| (also the leading dimension of A(I,J))
+    int capU;      // distinct candidates |U|
+    int capSort;   // raw index lists before sorting (a power of two)
+    int capPairs;  // candidates on the std::sort tie path
+};
+
+/* Routing capacities of the size bins (|J| and raw row-list length). */
+struct BinEdges {
+    int capJ[8];
+    int capSort[8];
 };
 
 /* Everything a column kernel needs; passed by value. */
@@ -57,8 +68,11 @@ struct BuildArgs {
     const int* queue;       // columns to build in this launch
     int nqueue;
     int* counter;           // persistent work-queue head
-    int* overflow;          // overflow list (columns that did not fit)
-    int* n_overflow;
+    int* overflow;          // columns that did not fit this launch's layout
+    int* n_overflow;        // their count
+    int* fallback;          // mixed path: columns to rebuild in FP64
+    int* n_fallback;
+    int status_or;          // status bits added to every column of this launch
     int* m_len;
     int* m_idx;
     double* m_val;
@@ -66,34 +80,77 @@ struct BuildArgs {
     int* nsolve;
     int* status;
     double* work;
-    char* scratch;
-    ScratchLayout L;
+    char* scratch;          // per-wave HBM working sets of the generic kernel
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
-/* The 64-lane reduction: partial sums are combined by the butterfly
- * p_l <- p_l + p_(l xor s), s = 32 .. 1; every lane ends with p_0. */
-__device__ __forceinline__ double wave_sum_fixed(double p)
-{
+/*
+ * Lane groups.  A wave runs 64 / G columns at once, one per group of G lanes
+ * (G = 16, 32 or 64).  Row r of a column lives on lane r % G of its group.
+ * For |I| <= G this reproduces the 64-lane order exactly: the 64-lane
+ * butterfly's first steps (offsets >= G) only add +0 from lanes that hold no
+ * row, which leaves every nonzero partial unchanged.  Larger columns run with
+ * G = 64.
+ */
+template <int G>
+struct Group {
+    static_assert(G == 16 || G == 32 || G == 64, "group width");
+    static __device__ __forceinline__ int lane() { return threadIdx.x & (G - 1); }
+    static __device__ __forceinline__ int index() { return (threadIdx.x & (kWave - 1)) / G; }
+    /* the fixed-order sum: p_l <- p_l + p_(l xor s), s = G/2 .. 1 */
+    static __device__ __forceinline__ double sum_fixed(double p)
+    {
 #pragma unroll
-    for (int s = kWave / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s, kWave);
-    return p;
-}
+        for (int s = G / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s, G);
+        return p;
+    }
+    static __device__ __forceinline__ float sum_f(float p)
+    {
+#pragma unroll
+        for (int s = G / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s, G);
+        return p;
+    }
+    static __device__ __forceinline__ double min(double p)
+    {
+#pragma unroll
+        for (int s = G / 2; s >= 1; s >>= 1) p = fmin(p, __shfl_xor(p, s, G));
+        return p;
+    }
+    static __device__ __forceinline__ int sum_int(int p)
+    {
+#pragma unroll
+        for (int s = G / 2; s >= 1; s >>= 1) p += __shfl_xor(p, s, G);
+        return p;
+    }
+    /* value of lane 0 of the group */
+    template <class T>
+    static __device__ __forceinline__ T bcast0(T v) { return __shfl(v, 0, G); }
+    /* inclusive prefix sum over the group */
+    static __device__ __forceinline__ int scan_incl(int v)
+    {
+        const int l = lane();
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            int u = __shfl_up(v, d, G);
+            if (l >= d) v += u;
+        }
+        return v;
+    }
+    /* ballot restricted to the group, bit i = group lane i */
+    static __device__ __forceinline__ unsigned long long ballot(bool pred)
+    {
+        unsigned long long m = __ballot(pred);
+        if (G == 64) return m;
+        return (m >> (index() * G)) & ((1ull << G) - 1ull);
+    }
+    static __device__ __forceinline__ unsigned long long lanes_below() { return (1ull << lane()) - 1ull; }
+};
 
-__device__ __forceinline__ double wave_min(double p)
-{
-#pragma unroll
-    for (int s = kWave / 2; s >= 1; s >>= 1) p = fmin(p, __shfl_xor(p, s, kWave));
-    return p;
-}
-
-__device__ __forceinline__ int wave_sum_int(int p)
-{
-#pragma unroll
-    for (int s = kWave / 2; s >= 1; s >>= 1) p += __shfl_xor(p, s, kWave);
-    return p;
-}
+/* Backwards-compatible whole-wave helpers. */
+__device__ __forceinline__ double wave_sum_fixed(double p) { return Group<64>::sum_fixed(p); }
+__device__ __forceinline__ double wave_min(double p) { return Group<64>::min(p); }
+__device__ __forceinline__ int wave_sum_int(int p) { return Group<64>::sum_int(p); }
 
 __device__ __forceinline__ int lower_bound_i(const int* a, int n, int key)
 {
@@ -110,24 +167,27 @@ __device__ __forceinline__ int lower_bound_i(const int* a, int n, int key)
 
 /* Sort buf[0..count) ascending (bitonic network over the next power of two,
  * padded with INT_MAX; buf must hold that many ints) and write the distinct
- * values to out.  Whole-wave call; returns the number of distinct values. */
-__device__ inline int wave_sort_unique(int* buf, int count, int* out)
+ * values to out (out may equal buf).  Whole-group call; the wave must be
+ * converged at the barriers.  Returns the number of distinct values. */
+template <int G>
+__device__ __forceinline__ int group_sort_unique(int* buf, int count, int* out)
 {
-    const int lane = lane_id();
+    using Gr = Group<G>;
+    const int l = Gr::lane();
     int N = 1;
     while (N < count) N <<= 1;
-    for (int i = count + lane; i < N; i += kWave) buf[i] = 0x7fffffff;
+    for (int i = count + l; i < N; i += G) buf[i] = 0x7fffffff;
     __syncthreads();
     for (int k = 2; k <= N; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < N; i += kWave) {
-                int l = i ^ j;
-                if (l > i) {
-                    int a = buf[i], b = buf[l];
+            for (int i = l; i < N; i += G) {
+                int p = i ^ j;
+                if (p > i) {
+                    int a = buf[i], b = buf[p];
                     bool up = (i & k) == 0;
                     if ((a > b) == up) {
                         buf[i] = b;
-                        buf[l] = a;
+                        buf[p] = a;
                     }
                 }
             }
@@ -135,18 +195,20 @@ __device__ inline int wave_sort_unique(int* buf, int count, int* out)
         }
     }
     int total = 0;
-    for (int base = 0; base < count; base += kWave) {
-        int i = base + lane;
+    for (int base = 0; base < count; base += G) {
+        int i = base + l;
         int v = i < count ? buf[i] : 0;
         bool keep = i < count && (i == 0 || buf[i - 1] != v);
-        unsigned long long mask = __ballot(keep);
-        int pos = total + __popcll(mask & ((1ull << lane) - 1ull));
+        unsigned long long mask = Gr::ballot(keep);
+        int pos = total + __popcll(mask & Gr::lanes_below());
         if (keep) out[pos] = v;
         total += __popcll(mask);
     }
     __syncthreads();
     return total;
 }
+
+__device__ inline int wave_sort_unique(int* buf, int count, int* out) { return group_sort_unique<64>(buf, count, out); }
 
 }  // namespace mspai
 

@@ -37,11 +37,21 @@ constexpr long kThreshold = 16;
 
 MSPAI_HD inline bool lt(const RhoPair& a, const RhoPair& b) { return a.rho < b.rho; }
 
+/* Element moves are written field by field: a whole-struct copy becomes a
+ * private-memory temporary in device code. */
+MSPAI_HD inline void move_to(RhoPair* dst, const RhoPair* src)
+{
+    dst->idx = src->idx;
+    dst->rho = src->rho;
+}
+
 MSPAI_HD inline void swap_at(RhoPair* a, RhoPair* b)
 {
-    RhoPair t = *a;
-    *a = *b;
-    *b = t;
+    int ti = a->idx;
+    double tr = a->rho;
+    move_to(a, b);
+    b->idx = ti;
+    b->rho = tr;
 }
 
 MSPAI_HD inline long floor_log2(long n)
@@ -54,33 +64,34 @@ MSPAI_HD inline long floor_log2(long n)
     return r;
 }
 
-MSPAI_HD inline void push_heap(RhoPair* first, long hole, long top, RhoPair value)
+MSPAI_HD inline void push_heap(RhoPair* first, long hole, long top, int vi, double vr)
 {
     long parent = (hole - 1) / 2;
-    while (hole > top && lt(first[parent], value)) {
-        first[hole] = first[parent];
+    while (hole > top && first[parent].rho < vr) {
+        move_to(&first[hole], &first[parent]);
         hole = parent;
         parent = (hole - 1) / 2;
     }
-    first[hole] = value;
+    first[hole].idx = vi;
+    first[hole].rho = vr;
 }
 
-MSPAI_HD inline void adjust_heap(RhoPair* first, long hole, long len, RhoPair value)
+MSPAI_HD inline void adjust_heap(RhoPair* first, long hole, long len, int vi, double vr)
 {
     const long top = hole;
     long child = hole;
     while (child < (len - 1) / 2) {
         child = 2 * (child + 1);
         if (lt(first[child], first[child - 1])) --child;
-        first[hole] = first[child];
+        move_to(&first[hole], &first[child]);
         hole = child;
     }
     if ((len & 1) == 0 && child == (len - 2) / 2) {
         child = 2 * (child + 1);
-        first[hole] = first[child - 1];
+        move_to(&first[hole], &first[child - 1]);
         hole = child - 1;
     }
-    push_heap(first, hole, top, value);
+    push_heap(first, hole, top, vi, vr);
 }
 
 MSPAI_HD inline void heap_sort(RhoPair* first, RhoPair* last)
@@ -89,17 +100,17 @@ MSPAI_HD inline void heap_sort(RhoPair* first, RhoPair* last)
     if (len >= 2) {
         long parent = (len - 2) / 2;
         while (true) {
-            RhoPair v = first[parent];
-            adjust_heap(first, parent, len, v);
+            adjust_heap(first, parent, len, first[parent].idx, first[parent].rho);
             if (parent == 0) break;
             --parent;
         }
     }
     while (last - first > 1) {
         --last;
-        RhoPair v = *last;
-        *last = *first;
-        adjust_heap(first, 0, last - first, v);
+        int vi = last->idx;
+        double vr = last->rho;
+        move_to(last, first);
+        adjust_heap(first, 0, last - first, vi, vr);
     }
 }
 
@@ -135,14 +146,16 @@ MSPAI_HD inline RhoPair* unguarded_partition(RhoPair* first, RhoPair* last, RhoP
 
 MSPAI_HD inline void unguarded_linear_insert(RhoPair* last)
 {
-    RhoPair v = *last;
+    int vi = last->idx;
+    double vr = last->rho;
     RhoPair* next = last - 1;
-    while (lt(v, *next)) {
-        *last = *next;
+    while (vr < next->rho) {
+        move_to(last, next);
         last = next;
         --next;
     }
-    *last = v;
+    last->idx = vi;
+    last->rho = vr;
 }
 
 MSPAI_HD inline void insertion_sort(RhoPair* first, RhoPair* last)
@@ -150,9 +163,11 @@ MSPAI_HD inline void insertion_sort(RhoPair* first, RhoPair* last)
     if (first == last) return;
     for (RhoPair* i = first + 1; i != last; ++i) {
         if (lt(*i, *first)) {
-            RhoPair v = *i;
-            for (RhoPair* p = i; p != first; --p) *p = *(p - 1);
-            *first = v;
+            int vi = i->idx;
+            double vr = i->rho;
+            for (RhoPair* p = i; p != first; --p) move_to(p, p - 1);
+            first->idx = vi;
+            first->rho = vr;
         } else {
             unguarded_linear_insert(i);
         }
@@ -160,33 +175,36 @@ MSPAI_HD inline void insertion_sort(RhoPair* first, RhoPair* last)
 }
 
 /* The recursion on the right part of each partition is turned into an
- * explicit stack; the order in which parts are finished does not change the
- * result because the parts are disjoint. */
-MSPAI_HD inline void introsort_loop(RhoPair* first, RhoPair* last, long depth)
+ * explicit stack of (first, last, depth) index triples held in `stack`
+ * (kStackFrames frames, caller-provided so device code keeps it out of
+ * private memory).  Frames on the stack have strictly decreasing depth, so at
+ * most 2*floor(log2 n) + 1 are live.  The order in which parts are finished
+ * does not change the result because the parts are disjoint. */
+constexpr int kStackFrames = 64;
+
+MSPAI_HD inline void introsort_loop(RhoPair* base, long n, long depth, int* stack)
 {
-    struct Frame {
-        RhoPair* first;
-        RhoPair* last;
-        long depth;
-    };
-    Frame stack[64];
     int sp = 0;
-    stack[sp++] = {first, last, depth};
+    stack[0] = 0;
+    stack[1] = (int)n;
+    stack[2] = (int)depth;
+    sp = 1;
     while (sp > 0) {
-        Frame f = stack[--sp];
-        RhoPair* lo = f.first;
-        RhoPair* hi = f.last;
-        long d = f.depth;
+        --sp;
+        long lo = stack[3 * sp], hi = stack[3 * sp + 1], d = stack[3 * sp + 2];
         while (hi - lo > kThreshold) {
             if (d == 0) {
-                heap_sort(lo, hi);
+                heap_sort(base + lo, base + hi);
                 break;
             }
             --d;
-            RhoPair* mid = lo + (hi - lo) / 2;
-            median_to_first(lo, lo + 1, mid, hi - 1);
-            RhoPair* cut = unguarded_partition(lo + 1, hi, lo);
-            stack[sp++] = {cut, hi, d};
+            long mid = lo + (hi - lo) / 2;
+            median_to_first(base + lo, base + lo + 1, base + mid, base + hi - 1);
+            long cut = unguarded_partition(base + lo + 1, base + hi, base + lo) - base;
+            stack[3 * sp] = (int)cut;
+            stack[3 * sp + 1] = (int)hi;
+            stack[3 * sp + 2] = (int)d;
+            ++sp;
             hi = cut;
         }
     }
@@ -194,12 +212,13 @@ MSPAI_HD inline void introsort_loop(RhoPair* first, RhoPair* last, long depth)
 
 }  // namespace introsort_detail
 
-/* Sort [first, last) exactly as libstdc++'s std::sort with a.rho < b.rho. */
-MSPAI_HD inline void std_sort_rho(RhoPair* first, RhoPair* last)
+/* Sort [first, last) exactly as libstdc++'s std::sort with a.rho < b.rho.
+ * `stack` holds 3 * introsort_detail::kStackFrames ints of scratch. */
+MSPAI_HD inline void std_sort_rho(RhoPair* first, RhoPair* last, int* stack)
 {
     using namespace introsort_detail;
     if (first == last) return;
-    introsort_loop(first, last, floor_log2(last - first) * 2);
+    introsort_loop(first, last - first, floor_log2(last - first) * 2, stack);
     if (last - first > kThreshold) {
         insertion_sort(first, first + kThreshold);
         for (RhoPair* i = first + kThreshold; i != last; ++i) unguarded_linear_insert(i);

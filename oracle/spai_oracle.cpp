@@ -375,6 +375,7 @@ void count_solve(const Problem& P, const Work& w, double* work)
 int solve_cached(const Problem& P, Work& w)
 {
     int m = (int)w.I.size(), n = (int)w.J.size();
+    if ((int)w.b.size() < n) w.b.resize(n, 0.0);  // |I| < |J|: room for the zero solution
     w.aqr = w.ahat;
     w.tau.assign(std::max(1, std::min(m, n)), 0.0);
     if (P.o->cache <= 0) return ls_solve(m, n, w.aqr.data(), w.tau.data(), w.b.data());

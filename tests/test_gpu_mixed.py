@@ -1,0 +1,69 @@
+"""Mixed precision (FP32 QR + FP64 refinement) against the FP64 oracle.
+
+Bar: the sparsity pattern of M is identical; every column's values are within
+1e-9 relative (2-norm over the column) of the oracle; the residual norms
+within 1e-9 relative; solve counts equal.  Columns the guards send to the
+FP64 kernels carry MSPAI_COL_FALLBACK and are bitwise equal to the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mspai_amd import gen
+from tests.util import csparse_problem
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+FALLBACK = 8
+TOL = 1e-9
+
+CASES = {
+    "c1_laplace2d_64": lambda: gen.config1(64),
+    "c2_convdiff_64": lambda: gen.config2(64),
+    "c2_convdiff_256": lambda: gen.config2(256),
+    "c3_random_16384": lambda: gen.config3(n=16384, bw=800),
+    "c4_probing_16": lambda: gen.config4(16),
+    "c5_varcoef_12": lambda: gen.config5(12),
+    "cs_fs_183_1": lambda: csparse_problem("fs_183_1", eps=0.2, mn=5, ns=6),
+    "cs_bcsstk01": lambda: csparse_problem("bcsstk01"),
+}
+
+
+def check_mixed(oracle, prob, M, info):
+    ref = oracle.spai(prob, threads=THREADS)
+    Mref = oracle.to_csc(ref, prob.n)
+    assert np.array_equal(Mref.indptr, M.indptr) and np.array_equal(Mref.indices, M.indices), "pattern differs"
+    assert np.array_equal(ref["nsolve"], info["nsolve"])
+    assert np.array_equal(ref["status"], info["status"] & ~FALLBACK)
+    d = M.data - Mref.data
+    colerr = np.sqrt(np.add.reduceat(d * d, Mref.indptr[:-1])) if M.nnz else np.zeros(0)
+    colnorm = np.sqrt(np.add.reduceat(Mref.data ** 2, Mref.indptr[:-1])) if M.nnz else np.zeros(0)
+    rel = colerr / np.maximum(colnorm, 1e-300)
+    assert rel.max(initial=0.0) <= TOL, rel.max()
+    rr = np.abs(info["res_norm"] - ref["res_norm"]) / np.maximum(ref["res_norm"], 1e-300)
+    assert rr.max(initial=0.0) <= TOL
+    fb = (info["status"] & FALLBACK) != 0
+    # columns rebuilt in FP64 are the oracle's bits
+    for k in np.nonzero(fb)[0][:2000]:
+        a, b = M.indptr[k], M.indptr[k + 1]
+        assert np.array_equal(M.data[a:b].view(np.int64), Mref.data[a:b].view(np.int64))
+    return rel.max(initial=0.0), fb.mean()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_matches_oracle(oracle, gpu, name):
+    prob = CASES[name]()
+    print(f"[mixed {name}] n={prob.n}", flush=True)
+    M, info, stats = gpu.spai(prob, precision=1)
+    worst, fbfrac = check_mixed(oracle, prob, M, info)
+    print(f"   worst column rel err {worst:.2e}, fallback fraction {fbfrac:.4f}, bins {stats['bins']}",
+          flush=True)
+
+
+def test_mixed_full_c3(oracle, gpu):
+    prob = gen.config3()
+    print(f"[mixed full {prob.name}] n={prob.n}", flush=True)
+    M, info, stats = gpu.spai(prob, precision=1)
+    worst, fbfrac = check_mixed(oracle, prob, M, info)
+    print(f"   worst column rel err {worst:.2e}, fallback fraction {fbfrac:.4f}", flush=True)

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP library against the CPU FP64 oracle, through the C ABI.
+
+FP64 path: pattern, values, residual norms, solve counts, status bits and
+work counters must be bitwise identical to the oracle (both use the same
+fixed operation order).  Inputs are the five workloads at test sizes and at
+full size where the oracle finishes in seconds, the CSparse fixture matrices,
+and edge cases (empty column, zero diagonal, duplicate columns, identity).
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from mspai_amd import gen
+from tests.util import column_lstsq, csparse_problem, ls_close, same_csc_bitwise
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def run_both(oracle, gpu, prob, cols=None, **opts):
+    okw = {k: v for k, v in opts.items() if k in ("qr", "cache", "epsilon", "maxnew", "max_impr_steps", "use_mean")}
+    ref = oracle.spai(prob, threads=THREADS, cols=cols, **okw)
+    M, info, stats = gpu.spai(prob, **opts)
+    return ref, M, info, stats
+
+
+def assert_bitwise(oracle, prob, ref, M, info):
+    Mref = oracle.to_csc(ref, prob.n)
+    assert same_csc_bitwise(Mref, M), "M differs from the oracle"
+    assert np.array_equal(ref["res_norm"].view(np.int64), info["res_norm"].view(np.int64))
+    assert np.array_equal(ref["nsolve"], info["nsolve"])
+    assert np.array_equal(ref["status"], info["status"])
+    assert np.array_equal(ref["work"], info["work"])
+
+
+CASES = {
+    "c1_laplace2d_64": lambda: gen.config1(64),
+    "c1_laplace2d_16": lambda: gen.config1(16),
+    "c2_convdiff_32": lambda: gen.config2(32),
+    "c2_convdiff_256": lambda: gen.config2(256),
+    "c3_random_8192": lambda: gen.config3(n=8192, bw=500),
+    "c4_probing_8": lambda: gen.config4(8),
+    "c4_probing_16": lambda: gen.config4(16),
+    "c5_varcoef_8": lambda: gen.config5(8),
+    "c5_varcoef_12": lambda: gen.config5(12),
+    "cs_west0067": lambda: csparse_problem("west0067"),
+    "cs_fs_183_1": lambda: csparse_problem("fs_183_1", eps=0.2, mn=5, ns=6),
+    "cs_bcsstk01": lambda: csparse_problem("bcsstk01"),
+    "cs_t1": lambda: csparse_problem("t1"),
+    "cs_ash219": lambda: csparse_problem("ash219"),
+    "cs_ibm32a": lambda: csparse_problem("ibm32a"),
+    "cs_fs_183_1_static": lambda: csparse_problem("fs_183_1", adaptive=False),
+    "cs_west0067_static": lambda: csparse_problem("west0067", adaptive=False),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_bitwise(oracle, gpu, name):
+    prob = CASES[name]()
+    print(f"[case {name}] n={prob.n}", flush=True)
+    ref, M, info, _ = run_both(oracle, gpu, prob)
+    assert_bitwise(oracle, prob, ref, M, info)
+
+
+@pytest.mark.parametrize("opts", [dict(epsilon=0.0), dict(maxnew=1), dict(use_mean=0),
+                                  dict(max_impr_steps=9, epsilon=0.05), dict(maxnew=12, max_impr_steps=2)])
+def test_fp64_option_variants(oracle, gpu, opts):
+    prob = gen.config3(n=4096, bw=300)
+    print(f"[options {opts}]", flush=True)
+    ref, M, info, _ = run_both(oracle, gpu, prob, **opts)
+    assert_bitwise(oracle, prob, ref, M, info)
+
+
+def _edge_matrix():
+    """Nonsymmetric 40x40 with: an empty column (NaN scores), a zero diagonal
+    (k not in I), two identical columns (exact zero pivot)."""
+    rng = np.random.Generator(np.random.PCG64(9))
+    n = 40
+    A = sp.random(n, n, density=0.12, random_state=np.random.RandomState(3), format="lil")
+    for i in range(n):
+        A[i, i] = 4.0 + rng.random()
+    A[:, 7] = 0.0
+    A[11, 11] = 0.0
+    A[:, 20] = A[:, 19]
+    A = sp.csc_matrix(A)
+    A.eliminate_zeros()
+    return gen._csc(*sp.find(A)[:2], sp.find(A)[2], (n, n))
+
+
+def test_fp64_edge_cases(oracle, gpu):
+    A = _edge_matrix()
+    n = A.shape[1]
+    for P, ns in ((gen.diag_pattern(n), 4), (gen.pattern_of(A), 0)):
+        P = sp.csc_matrix(P)
+        # every column of the start pattern must be nonempty for the reference
+        P = P + gen.diag_pattern(n)
+        P = gen.pattern_of(P)
+        prob = gen.Problem("edge", A, P, None, epsilon=0.1, maxnew=3, max_impr_steps=ns)
+        ref, M, info, _ = run_both(oracle, gpu, prob)
+        assert_bitwise(oracle, prob, ref, M, info)
+        assert np.any(info["status"] != 0)
+
+
+def test_identity(oracle, gpu):
+    n = 300
+    prob = gen.Problem("eye", gen._csc(np.arange(n), np.arange(n), np.ones(n), (n, n)),
+                       gen.diag_pattern(n), None, epsilon=0.0, maxnew=3, max_impr_steps=3)
+    ref, M, info, _ = run_both(oracle, gpu, prob)
+    assert_bitwise(oracle, prob, ref, M, info)
+    assert same_csc_bitwise(M, sp.identity(n, format="csc"))
+
+
+def test_binning_off_matches(oracle, gpu):
+    """The generic kernel alone (no size bins) gives the same bits."""
+    prob = gen.config3(n=8192, bw=500)
+    ref, M, info, _ = run_both(oracle, gpu, prob, binning=0)
+    assert_bitwise(oracle, prob, ref, M, info)
+
+
+def test_repeat_build_is_deterministic(gpu):
+    prob = gen.config3(n=16384, bw=800)
+    M1, i1, _ = gpu.spai(prob)
+    M2, i2, _ = gpu.spai(prob)
+    assert same_csc_bitwise(M1, M2)
+    assert np.array_equal(i1["nsolve"], i2["nsolve"])
+
+
+def test_full_c2_and_c3_bitwise(oracle, gpu):
+    """The two adaptive workloads at full size (n = 65,536 and 262,144)."""
+    for prob in (gen.config2(), gen.config3()):
+        print(f"[full {prob.name}] n={prob.n}", flush=True)
+        ref, M, info, _ = run_both(oracle, gpu, prob)
+        assert_bitwise(oracle, prob, ref, M, info)
+
+
+def test_full_c4_bitwise(oracle, gpu):
+    prob = gen.config4()
+    print(f"[full {prob.name}] n={prob.n}", flush=True)
+    ref, M, info, _ = run_both(oracle, gpu, prob)
+    assert_bitwise(oracle, prob, ref, M, info)
+
+
+def test_full_c5_sampled_columns(oracle, gpu):
+    """n = 2,097,152: the GPU builds every column; the oracle re-solves a
+    stratified sample and the sampled columns must agree bitwise, and every
+    column's pattern is exactly P(A^2)."""
+    prob = gen.config5()
+    print(f"[full {prob.name}] n={prob.n}", flush=True)
+    M, info, stats = gpu.spai(prob)
+    assert np.array_equal(M.indptr, prob.P.indptr) and np.array_equal(M.indices, prob.P.indices)
+    rng = np.random.Generator(np.random.PCG64(3))
+    cols = np.sort(rng.choice(prob.n, size=8192, replace=False)).astype(np.int32)
+    ref = oracle.spai(prob, threads=THREADS, cols=cols)
+    for t, k in enumerate(cols):
+        a, b = M.indptr[k], M.indptr[k + 1]
+        assert np.array_equal(M.indices[a:b], ref["idx"][t, :ref["len"][t]])
+        assert np.array_equal(M.data[a:b].view(np.int64), ref["val"][t, :ref["len"][t]].view(np.int64))
+    assert np.array_equal(info["res_norm"][cols].view(np.int64), ref["res_norm"].view(np.int64))
+    # size-independent property: every column solves its least-squares problem
+    for k in cols[:64]:
+        J = M.indices[M.indptr[k]:M.indptr[k + 1]]
+        I, Ah, bb, x = column_lstsq(prob.A, J, k)
+        assert ls_close(Ah, bb, M.data[M.indptr[k]:M.indptr[k + 1]], x)

@@ -66,6 +66,7 @@ typedef struct {
     int ncols;
     int n_fallback;         /* mixed path: columns recomputed in FP64 */
     int n_overflow;         /* columns that outgrew their size bin and moved on */
+    int n_cache_hits;       /* least-squares solves served by the pattern->R cache */
 } mspai_stats;
 
 /* Context on HIP device `device` (replaces the MPI world of Com_Server). */
@@ -111,6 +112,33 @@ int mspai_download_csc(mspai_ctx* ctx, int* colptr, int* rowind, double* val);
  * Any pointer may be NULL. */
 int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int* nsolve, int* status,
                                double* work);
+
+/*
+ * Device-pointer entry points for callers that keep their arrays on the GPU
+ * (the PyTorch-ROCm test harness passes tensor storage).  No counterpart in
+ * the reference, which has no device boundary.
+ */
+/* Replace C's values (nnz doubles, same pattern) from device memory. */
+int mspai_update_values_device(mspai_ctx* ctx, const double* d_val);
+/* Replace the target's values (same pattern) from device memory. */
+int mspai_update_target_device(mspai_ctx* ctx, const double* d_val);
+/* Copy the values of the last built M (nnz doubles, CSC order) to device memory. */
+int mspai_result_values_device(mspai_ctx* ctx, double* d_out);
+
+/*
+ * Adjoint of the last build with its pattern held fixed: given
+ * d_gM = dL/dM (device, nnz(M) doubles in M's CSC order) write
+ * d_gC = dL/dC (device, nnz(C) doubles in C's CSC order) and, when a target
+ * is set and d_gB is not NULL, d_gB = dL/dB (nnz(B) doubles).  Per column:
+ * r = b - C(I,J) m, R^T R lambda = g, dL/dC(i,j) = r_i lambda_j -
+ * (C lambda)_i m_j, dL/db = C lambda, with the QR recomputed (FP64, or FP32
+ * with refinement when opts->precision is 1).  No counterpart in the
+ * reference (it has no derivatives); MSPAI probing weights rho enter through
+ * C's and B's probe rows, so dL/drho follows from gC and gB by the chain rule.
+ * Accumulation into gC uses atomics, so its last bits depend on timing.
+ */
+int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const double* d_gM, double* d_gC, double* d_gB,
+                   mspai_stats* stats);
 
 /* Sums over all columns of the 8 work counters of the last build:
  * QR flop, other solve flop (Q^T b, triangular solve, residual), gather

@@ -22,7 +22,9 @@ COL_SINGULAR, COL_UNDERDET, COL_NAN_RHO, COL_FALLBACK, COL_K_NOT_IN_I = 1, 2, 4,
 EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspai_last_error",
             "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_set_target",
             "mspai_build", "mspai_result_nnz", "mspai_download_csc",
-            "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts"]
+            "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts",
+            "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
+            "mspai_backward"]
 
 
 class Opts(C.Structure):
@@ -33,7 +35,8 @@ class Opts(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("solve_ms", C.c_double), ("nnz", C.c_longlong),
-                ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int)]
+                ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int),
+                ("n_cache_hits", C.c_int)]
 
 
 _lib = None
@@ -139,7 +142,32 @@ class Builder:
         st = Stats()
         self._chk(self.lib.mspai_build(self.h, C.byref(o), C.byref(st)), "mspai_build")
         return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz,
-                "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow}
+                "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow,
+                "n_cache_hits": st.n_cache_hits}
+
+    # ---- device-pointer entry points (addresses of device buffers, e.g. tensor storage)
+    def update_values_device(self, ptr: int):
+        self._chk(self.lib.mspai_update_values_device(self.h, C.c_void_p(ptr)), "mspai_update_values_device")
+
+    def update_target_device(self, ptr: int):
+        self._chk(self.lib.mspai_update_target_device(self.h, C.c_void_p(ptr)), "mspai_update_target_device")
+
+    def result_values_device(self, ptr: int):
+        self._chk(self.lib.mspai_result_values_device(self.h, C.c_void_p(ptr)), "mspai_result_values_device")
+
+    def backward_device(self, gm_ptr: int, gc_ptr: int, gb_ptr=None, precision=0, binning=1):
+        o = Opts(0.0, 0, 0, 1, 0, 0, precision, binning, 0)
+        st = Stats()
+        self._chk(self.lib.mspai_backward(self.h, C.byref(o), C.c_void_p(gm_ptr), C.c_void_p(gc_ptr),
+                                          C.c_void_p(gb_ptr) if gb_ptr else None, C.byref(st)),
+                  "mspai_backward")
+        return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "n_fallback": st.n_fallback,
+                "n_overflow": st.n_overflow}
+
+    def result_nnz(self) -> int:
+        nnz = C.c_longlong()
+        self._chk(self.lib.mspai_result_nnz(self.h, C.byref(nnz)), "mspai_result_nnz")
+        return nnz.value
 
     def work_totals(self):
         w = np.zeros(NWORK, np.float64)

@@ -122,7 +122,18 @@ struct mspai_ctx {
     void* d_cub = nullptr;
     size_t cub_bytes = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    /* pattern->R cache */
+    int cache_slots = 0, cache_capI = 0, cache_capJ = 0;
+    int* d_cache_state = nullptr;
+    uint32_t* d_cache_key = nullptr;
+    int* d_cache_m = nullptr;
+    int* d_cache_n = nullptr;
+    int* d_cache_status = nullptr;
+    double* d_cache_data = nullptr;
     int last_nbins = 0;
+    bool last_mixed = false;
+    int last_binning = 1;
+    long long nnzB = 0;
     int last_routed[8] = {0};
     int last_queued[8] = {0};
 };
@@ -226,6 +237,12 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_counter);
     dfree(ctx->d_wsum);
     dfree(ctx->d_scratch);
+    dfree(ctx->d_cache_state);
+    dfree(ctx->d_cache_key);
+    dfree(ctx->d_cache_m);
+    dfree(ctx->d_cache_n);
+    dfree(ctx->d_cache_status);
+    dfree(ctx->d_cache_data);
     if (ctx->d_cub) (void)hipFree(ctx->d_cub);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -349,6 +366,7 @@ extern "C" int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* ro
         dfree(ctx->d_b_ptr);
         dfree(ctx->d_b_ind);
         dfree(ctx->d_b_val);
+        ctx->nnzB = 0;
         return MSPAI_OK;
     }
     if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "set_target: upload the matrix first");
@@ -365,6 +383,7 @@ extern "C" int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* ro
     if ((rc = upload(ctx, ctx->d_b_ind, rowind, (size_t)colptr[n]))) return rc;
     if ((rc = upload(ctx, ctx->d_b_val, val, (size_t)colptr[n]))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nnzB = colptr[n];
     return MSPAI_OK;
 }
 
@@ -379,6 +398,8 @@ static int check_opts(mspai_ctx* ctx, const mspai_opts* o)
     if (o->qr < 0 || o->qr > 5) return fail(ctx, MSPAI_ERR_ARG, "Illegal parameter for qr");
     if (o->qr == 1 || o->qr > 2)
         return fail(ctx, MSPAI_ERR_UNSUPPORTED, "qr 1/3/4/5 select the sparse QR variants, which this library does not build");
+    if (o->qr == 2 && o->precision == 1)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED, "the QR update (-qr 2) runs in FP64 only");
     if (o->cache > 0 && o->qr > 0)
         return fail(ctx, MSPAI_ERR_ARG, "QR-Updates and Caching optimization cannot be used simultaneously");
     if (o->use_mean < 0 || o->use_mean > 1) return fail(ctx, MSPAI_ERR_ARG, "Illegal value for using mean value");
@@ -475,6 +496,16 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.bq32 = take(capB * 4);
     L.g = take((size_t)capJ * 8);
     L.y = take((size_t)capJ * 8);
+    L.z = take((size_t)capJ * 8);
+    L.srow = take(capB * 4);
+    L.spos = take(capB * 4);
+    L.iold = take(capB * 4);
+    L.sposold = take(capB * 4);
+    L.jold = take((size_t)capJ * 4);
+    L.colid = take((size_t)capJ * 4);
+    L.reflrows = take((size_t)capJ * 4);
+    L.bperm = take(capB * 8);
+    L.yq = take(capB * 8);
     L.per_wave = align_up(o, 256);
     return L;
 }
@@ -535,6 +566,150 @@ static int ensure_scratch(mspai_ctx* ctx, size_t need)
     return MSPAI_OK;
 }
 
+/* Launch routing and the bin chain over the columns whose patterns are in
+ * a.p_ptr/p_ind.  Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb =
+ * generic), mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills
+ * the first queues; a bin's overflow appends to the next queue of its chain;
+ * the mixed chain ends in FP64 queue 0, which also takes the columns the
+ * mixed guards decline.  Counters: FP64 [0..8) lengths, [8..16) heads,
+ * [16..24) routed lengths, mixed [24..32) lengths, [32..40) heads. */
+static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins, const ScratchLayout& G,
+                     int gwaves, bool mixed, int fp64_mode, int n)
+{
+    const int nb = (int)bins.size();
+    BinEdges edges{};
+    for (int b = 0; b < nb; ++b) {
+        edges.capJ[b] = bins[b].L.capJ;
+        edges.capSort[b] = bins[b].L.capSort;
+    }
+    int* counts = ctx->d_counter;
+    int* heads = ctx->d_counter + 8;
+    int* mcounts = ctx->d_counter + 24;
+    int* mheads = ctx->d_counter + 32;
+    int* fq = ctx->d_queues;
+    int* mq = ctx->d_queues + (size_t)(kMaxBins + 1) * n;
+    HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 64 * sizeof(int), ctx->stream));
+    a.status_or = 0;
+    if (mixed) {
+        launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->stream);
+        HIPCHK(hipGetLastError());
+        launch_copy_ints(mcounts, ctx->d_counter + 16, nb, ctx->stream);
+        launch_copy_ints(counts, ctx->d_counter + 16 + nb, 1, ctx->stream);
+        a.fallback = fq;
+        a.n_fallback = counts;
+        for (int b = 0; b < nb; ++b) {
+            const ScratchLayout& L = bins[b].L;
+            const int grp = bins[b].group;
+            a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
+            a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
+            size_t lds = L.per_wave * (64 / grp);
+            int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+            launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
+                          ctx->num_cu * per_cu, ctx->stream);
+            HIPCHK(hipGetLastError());
+        }
+        a.status_or = COL_FALLBACK;
+    } else {
+        launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->stream);
+        HIPCHK(hipGetLastError());
+        launch_copy_ints(counts, ctx->d_counter + 16, nb + 1, ctx->stream);
+    }
+    a.fallback = nullptr;
+    a.n_fallback = nullptr;
+    for (int b = 0; b < nb; ++b) {
+        const ScratchLayout& L = bins[b].L;
+        const int grp = bins[b].group;
+        a.overflow = fq + (size_t)(b + 1) * n;
+        a.n_overflow = counts + b + 1;
+        size_t lds = L.per_wave * (64 / grp);
+        int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+        launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
+                      ctx->num_cu * per_cu, ctx->stream);
+        HIPCHK(hipGetLastError());
+    }
+    a.overflow = fq;  // never written: the generic layout holds every column
+    a.n_overflow = counts + 7;
+    launch_fp64_generic(a, G, fp64_mode, fq + (size_t)nb * n, counts + nb, heads + nb, gwaves, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return MSPAI_OK;
+}
+
+/* Read the queue counters of the last chain into stats and the context. */
+static int read_chain_counts(mspai_ctx* ctx, int nbins, bool mixed, mspai_stats* stats)
+{
+    int cnt[64];
+    HIPCHK(hipMemcpy(cnt, ctx->d_counter, sizeof(cnt), hipMemcpyDeviceToHost));
+    ctx->last_nbins = nbins;
+    stats->n_overflow = 0;
+    stats->n_fallback = 0;
+    if (mixed) {
+        int moved = 0;
+        for (int b = 1; b < nbins; ++b) moved += cnt[24 + b] - cnt[16 + b];
+        stats->n_overflow = moved;
+        stats->n_fallback = cnt[0] - cnt[16 + nbins];
+        for (int b = 0; b < 8; ++b) {
+            ctx->last_routed[b] = cnt[16 + b];
+            ctx->last_queued[b] = b < nbins ? cnt[24 + b] : (b == nbins ? cnt[0] : 0);
+        }
+    } else {
+        for (int b = 0; b <= nbins; ++b) stats->n_overflow += cnt[b] - cnt[16 + b];
+        for (int b = 0; b < 8; ++b) {
+            ctx->last_routed[b] = cnt[16 + b];
+            ctx->last_queued[b] = cnt[b];
+        }
+    }
+    return MSPAI_OK;
+}
+
+/* Slots of the pattern->R cache: -cs entries rounded up to a power of two,
+ * at most 2^18; each slot holds up to 32 x 16 blocks. */
+static int ensure_cache(mspai_ctx* ctx, int entries, int capI, int capJ)
+{
+    int slots = 1;
+    while (slots < entries && slots < (1 << 18)) slots <<= 1;
+    if (slots == ctx->cache_slots && capI == ctx->cache_capI && capJ == ctx->cache_capJ) return MSPAI_OK;
+    dfree(ctx->d_cache_state);
+    dfree(ctx->d_cache_key);
+    dfree(ctx->d_cache_m);
+    dfree(ctx->d_cache_n);
+    dfree(ctx->d_cache_status);
+    dfree(ctx->d_cache_data);
+    size_t sd = 2 * (size_t)capI * capJ + 2 * (size_t)capJ + (size_t)capI;
+    HIPCHK(hipMalloc((void**)&ctx->d_cache_state, slots * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&ctx->d_cache_key, slots * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&ctx->d_cache_m, slots * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&ctx->d_cache_n, slots * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&ctx->d_cache_status, slots * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&ctx->d_cache_data, slots * sd * sizeof(double)));
+    ctx->cache_slots = slots;
+    ctx->cache_capI = capI;
+    ctx->cache_capJ = capJ;
+    return MSPAI_OK;
+}
+
+/* Generic-kernel sizing: keep its HBM working sets within 16 GiB. */
+static int prepare_generic(mspai_ctx* ctx, const ScratchLayout& G, int* gwaves)
+{
+    int w = std::max(1, ctx->num_cu) * 4;
+    const size_t budget = (size_t)16 << 30;
+    while (w > 1 && (size_t)w * G.per_wave > budget) w /= 2;
+    if ((size_t)w * G.per_wave > budget) return fail(ctx, MSPAI_ERR_CAPACITY, "per-column working set too large");
+    *gwaves = w;
+    return ensure_scratch(ctx, (size_t)w * G.per_wave);
+}
+
+static BuildArgs base_args(mspai_ctx* ctx)
+{
+    BuildArgs a{};
+    a.A = DevCsc{ctx->nrows, ctx->ncols, ctx->d_colptr, ctx->d_rowind, ctx->d_val};
+    a.rp_ptr = ctx->d_rp_ptr;
+    a.rp_ind = ctx->d_rp_ind;
+    a.B = DevCsc{ctx->nrows, ctx->ncols, ctx->d_b_ptr, ctx->d_b_ind, ctx->d_b_val};
+    a.ajn = ctx->d_ajn;
+    a.scratch = ctx->d_scratch;
+    return a;
+}
+
 extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats)
 {
     if (!ctx || !opts) return MSPAI_ERR_ARG;
@@ -554,26 +729,14 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive);
     std::vector<Bin> bins;
     if (opts->binning) bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G);
-    /* the generic kernel: keep its HBM working sets within 16 GiB */
-    int gwaves = std::max(1, ctx->num_cu) * 4;
-    const size_t budget = (size_t)16 << 30;
-    while (gwaves > 1 && (size_t)gwaves * G.per_wave > budget) gwaves /= 2;
-    if ((size_t)gwaves * G.per_wave > budget) return fail(ctx, MSPAI_ERR_CAPACITY, "build: per-column working set too large");
-    if ((rc = ensure_scratch(ctx, (size_t)gwaves * G.per_wave))) return rc;
+    int gwaves = 0;
+    if ((rc = prepare_generic(ctx, G, &gwaves))) return rc;
 
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    DevCsc A{ctx->nrows, ctx->ncols, ctx->d_colptr, ctx->d_rowind, ctx->d_val};
-    hipLaunchKernelGGL(column_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A, ctx->d_ajn);
-    HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 64 * sizeof(int), ctx->stream));
-
-    BuildArgs a{};
-    a.A = A;
-    a.rp_ptr = ctx->d_rp_ptr;
-    a.rp_ind = ctx->d_rp_ind;
+    BuildArgs a = base_args(ctx);
+    hipLaunchKernelGGL(column_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, a.A, ctx->d_ajn);
     a.p_ptr = ctx->d_p_ptr;
     a.p_ind = ctx->d_p_ind;
-    a.B = DevCsc{ctx->nrows, ctx->ncols, ctx->d_b_ptr, ctx->d_b_ind, ctx->d_b_val};
-    a.ajn = ctx->d_ajn;
     a.epsilon = opts->epsilon;
     a.maxnew = opts->maxnew;
     a.ns = ns;
@@ -586,70 +749,25 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     a.nsolve = ctx->d_nsolve;
     a.status = ctx->d_status;
     a.work = ctx->d_work;
-    a.scratch = ctx->d_scratch;
-    a.status_or = 0;
-
+    const bool qrup = opts->qr == 2 && ns > 0 && opts->maxnew > 0;  // opt_level 3, src/imspai.cpp:483-491
+    const bool mixed = opts->precision == 1 && !bins.empty() && !qrup;
+    if (opts->cache > 0 && !qrup && opts->precision == 0) {
+        const int cI = std::min(G.capI, 32), cJ = std::min(cap, 16);
+        if ((rc = ensure_cache(ctx, opts->cache, cI, cJ))) return rc;
+        HIPCHK(hipMemsetAsync(ctx->d_cache_state, 0, ctx->cache_slots * sizeof(int), ctx->stream));
+        a.cache_slots = ctx->cache_slots;
+        a.cache_capI = ctx->cache_capI;
+        a.cache_capJ = ctx->cache_capJ;
+        a.cache_state = ctx->d_cache_state;
+        a.cache_key = ctx->d_cache_key;
+        a.cache_m = ctx->d_cache_m;
+        a.cache_n = ctx->d_cache_n;
+        a.cache_status = ctx->d_cache_status;
+        a.cache_data = ctx->d_cache_data;
+        a.cache_hits = ctx->d_counter + 48;
+    }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    /* Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb = generic),
-     * mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills the
-     * first queues; a bin's overflow appends to the next queue of its chain;
-     * the mixed chain ends in FP64 queue 0, which also takes the columns the
-     * mixed guards decline. */
-    const int nb = (int)bins.size();
-    const bool mixed = opts->precision == 1 && nb > 0;
-    BinEdges edges{};
-    for (int b = 0; b < nb; ++b) {
-        edges.capJ[b] = bins[b].L.capJ;
-        edges.capSort[b] = bins[b].L.capSort;
-    }
-    int* counts = ctx->d_counter;
-    int* heads = ctx->d_counter + 8;
-    int* mcounts = ctx->d_counter + 24;
-    int* mheads = ctx->d_counter + 32;
-    int* fq = ctx->d_queues;
-    int* mq = ctx->d_queues + (size_t)(kMaxBins + 1) * n;
-    if (mixed) {
-        /* routing into the mixed chain; "generic" (nb) lands in FP64 queue 0 */
-        launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->stream);
-        HIPCHK(hipGetLastError());
-        launch_copy_ints(mcounts, ctx->d_counter + 16, nb, ctx->stream);
-        launch_copy_ints(counts, ctx->d_counter + 16 + nb, 1, ctx->stream);
-        a.fallback = fq;
-        a.n_fallback = counts;
-        for (int b = 0; b < nb; ++b) {
-            const ScratchLayout& L = bins[b].L;
-            const int grp = bins[b].group;
-            a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
-            a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
-            size_t lds = L.per_wave * (64 / grp);
-            int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
-            launch_binned(a, L, grp, true, mq + (size_t)b * n, mcounts + b, mheads + b, ctx->num_cu * per_cu,
-                          ctx->stream);
-            HIPCHK(hipGetLastError());
-        }
-        a.status_or = COL_FALLBACK;
-    } else {
-        launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->stream);
-        HIPCHK(hipGetLastError());
-        launch_copy_ints(counts, ctx->d_counter + 16, nb + 1, ctx->stream);
-    }
-    a.fallback = nullptr;
-    a.n_fallback = nullptr;
-    for (int b = 0; b < nb; ++b) {
-        const ScratchLayout& L = bins[b].L;
-        const int grp = bins[b].group;
-        a.overflow = fq + (size_t)(b + 1) * n;
-        a.n_overflow = counts + b + 1;
-        size_t lds = L.per_wave * (64 / grp);
-        int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
-        launch_binned(a, L, grp, false, fq + (size_t)b * n, counts + b, heads + b, ctx->num_cu * per_cu,
-                      ctx->stream);
-        HIPCHK(hipGetLastError());
-    }
-    a.overflow = fq;  // never written: the generic layout holds every column
-    a.n_overflow = counts + 7;
-    launch_fp64_generic(a, G, fq + (size_t)nb * n, counts + nb, heads + nb, gwaves, ctx->stream);
-    HIPCHK(hipGetLastError());
+    if ((rc = run_chain(ctx, a, bins, G, gwaves, mixed, qrup ? 4 : 0, n))) return rc;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
 
     /* compaction: colptr[0] = 0, colptr[1..n] = inclusive scan of the lengths */
@@ -666,6 +784,8 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     int last = 0;
     HIPCHK(hipMemcpy(&last, ctx->d_Mptr + n, sizeof(int), hipMemcpyDeviceToHost));
     ctx->nnzM = last;
+    ctx->last_mixed = mixed;
+    ctx->last_binning = opts->binning;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float t_all = 0, t_solve = 0;
@@ -675,30 +795,91 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         stats->solve_ms = t_solve;
         stats->nnz = last;
         stats->ncols = n;
-        int cnt[64];
-        HIPCHK(hipMemcpy(cnt, ctx->d_counter, sizeof(cnt), hipMemcpyDeviceToHost));
-        const int nbb = (int)bins.size();
-        ctx->last_nbins = nbb;
-        if (mixed) {
-            /* FP64 queue 0 = routed-to-generic + last-bin overflow + guard declines */
-            int moved = 0;
-            for (int b = 1; b < nbb; ++b) moved += cnt[24 + b] - cnt[16 + b];
-            stats->n_overflow = moved;
-            stats->n_fallback = cnt[0] - cnt[16 + nbb];
-            for (int b = 0; b < 8; ++b) {
-                ctx->last_routed[b] = cnt[16 + b];
-                ctx->last_queued[b] = b < nbb ? cnt[24 + b] : (b == nbb ? cnt[0] : 0);
-            }
-        } else {
-            for (int b = 0; b <= nbb; ++b) stats->n_overflow += cnt[b] - cnt[16 + b];
-            for (int b = 0; b < 8; ++b) {
-                ctx->last_routed[b] = cnt[16 + b];
-                ctx->last_queued[b] = cnt[b];
-            }
-        }
+        if ((rc = read_chain_counts(ctx, (int)bins.size(), mixed, stats))) return rc;
+        int hits = 0;
+        HIPCHK(hipMemcpy(&hits, ctx->d_counter + 48, sizeof(int), hipMemcpyDeviceToHost));
+        stats->n_cache_hits = a.cache_slots > 0 ? hits : 0;
     }
     return MSPAI_OK;
 }
+
+extern "C" int mspai_update_values_device(mspai_ctx* ctx, const double* d_val)
+{
+    if (!ctx || !d_val || !ctx->d_val) return MSPAI_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(ctx->d_val, d_val, ctx->nnz * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nnzM = -1;
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_update_target_device(mspai_ctx* ctx, const double* d_val)
+{
+    if (!ctx || !d_val) return MSPAI_ERR_ARG;
+    if (!ctx->d_b_ptr) return fail(ctx, MSPAI_ERR_STATE, "update_target: set a target first");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(ctx->d_b_val, d_val, ctx->nnzB * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nnzM = -1;
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_result_values_device(mspai_ctx* ctx, double* d_out)
+{
+    if (!ctx || !d_out) return MSPAI_ERR_ARG;
+    if (ctx->nnzM < 0) return fail(ctx, MSPAI_ERR_STATE, "result: build first");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(d_out, ctx->d_Mval, ctx->nnzM * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const double* d_gM, double* d_gC,
+                              double* d_gB, mspai_stats* stats)
+{
+    if (!ctx || !opts || !d_gM || !d_gC) return MSPAI_ERR_ARG;
+    if (ctx->nnzM < 0) return fail(ctx, MSPAI_ERR_STATE, "backward: build M first (values unchanged since)");
+    if (d_gB && !ctx->d_b_ptr) return fail(ctx, MSPAI_ERR_ARG, "backward: gB given but no target set");
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc;
+    const int n = ctx->ncols;
+    const int cap = ctx->cap;
+    const ScratchLayout G = generic_layout(ctx, cap, 1, false);
+    std::vector<Bin> bins;
+    if (opts->binning) bins = bin_layouts(ctx, cap, 1, false, G);
+    int gwaves = 0;
+    if ((rc = prepare_generic(ctx, G, &gwaves))) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(hipMemsetAsync(d_gC, 0, ctx->nnz * sizeof(double), ctx->stream));
+    if (d_gB) HIPCHK(hipMemsetAsync(d_gB, 0, ctx->nnzB * sizeof(double), ctx->stream));
+    BuildArgs a = base_args(ctx);
+    a.p_ptr = ctx->d_Mptr;
+    a.p_ind = ctx->d_Mind;
+    a.m_in = ctx->d_Mval;
+    a.gM = d_gM;
+    a.gC = d_gC;
+    a.gB = d_gB;
+    a.cap = cap;
+    const bool mixed = opts->precision == 1 && !bins.empty();
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    if ((rc = run_chain(ctx, a, bins, G, gwaves, mixed, 2, n))) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float t_all = 0, t_solve = 0;
+        HIPCHK(hipEventElapsedTime(&t_all, ctx->ev[0], ctx->ev[3]));
+        HIPCHK(hipEventElapsedTime(&t_solve, ctx->ev[1], ctx->ev[2]));
+        stats->build_ms = t_all;
+        stats->solve_ms = t_solve;
+        stats->nnz = ctx->nnz;
+        stats->ncols = n;
+        if ((rc = read_chain_counts(ctx, (int)bins.size(), mixed, stats))) return rc;
+    }
+    return MSPAI_OK;
+}
+
 extern "C" int mspai_work_totals(mspai_ctx* ctx, double* totals)
 {
     if (!ctx || !totals) return MSPAI_ERR_ARG;

@@ -38,7 +38,8 @@ struct DevCsc {
 /* Per-wave working-set layout (byte offsets) and the sizes it holds.  One
  * layout per size bin; the generic layout is sized for the worst case. */
 struct ScratchLayout {
-    size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y;
+    size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y, z;
+    size_t srow, spos, iold, sposold, jold, colid, reflrows, bperm, yq;
     size_t per_wave;
     int capJ;      // |J|
     int capI;      // |I| (also the leading dimension of A(I,J))
@@ -73,6 +74,19 @@ struct BuildArgs {
     int* fallback;          // mixed path: columns to rebuild in FP64
     int* n_fallback;
     int status_or;          // status bits added to every column of this launch
+    const double* m_in;     // adjoint: values of M (pattern in p_ptr/p_ind)
+    const double* gM;       // adjoint: dL/dM, indexed like m_in
+    double* gC;             // adjoint: dL/dC, indexed like A.val (accumulated)
+    double* gB;             // adjoint: dL/dB, indexed like B.val (may be null)
+    int cache_slots;        // pattern->R cache: slots (power of two), 0 = off
+    int cache_capI, cache_capJ;
+    int* cache_state;       // 0 empty, 1 being written, 2 ready
+    uint32_t* cache_key;
+    int* cache_m;
+    int* cache_n;
+    int* cache_status;
+    double* cache_data;
+    int* cache_hits;        // number of solves served from the cache
     int* m_len;
     int* m_idx;
     double* m_val;

@@ -45,7 +45,18 @@ struct WaveScratch {
     float* aq32;  // mixed path: FP32 copy of A(I,J) (aliases aqr)
     float* bq32;  // mixed path: FP32 right-hand side
     double* g;    // mixed path: A(I,J)^T r
-    double* y;    // mixed path: triangular-solve work vector
+    double* y;    // mixed path: triangular-solve work vector; backward: m
+    double* z;    // backward: copy of dL/dm
+    /* QR-update path: storage order of rows / columns of the kept factor */
+    int* srow;     // storage row -> row index of A
+    int* spos;     // position in sorted I -> storage row
+    int* iold;     // previous sorted I
+    int* sposold;  // previous spos
+    int* jold;     // previous J
+    int* colid;    // storage column -> column index of A
+    int* reflrows; // rows a reflector spans (rows present when it was made)
+    double* bperm; // right-hand side in storage row order
+    double* yq;    // Q^T b work vector in storage row order
 };
 
 __device__ __forceinline__ WaveScratch wave_scratch(char* ws, const ScratchLayout& L)
@@ -68,11 +79,24 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, const ScratchLayou
     s.bq32 = (float*)(ws + L.bq32);
     s.g = (double*)(ws + L.g);
     s.y = (double*)(ws + L.y);
+    s.z = (double*)(ws + L.z);
+    s.srow = (int*)(ws + L.srow);
+    s.spos = (int*)(ws + L.spos);
+    s.iold = (int*)(ws + L.iold);
+    s.sposold = (int*)(ws + L.sposold);
+    s.jold = (int*)(ws + L.jold);
+    s.colid = (int*)(ws + L.colid);
+    s.reflrows = (int*)(ws + L.reflrows);
+    s.bperm = (double*)(ws + L.bperm);
+    s.yq = (double*)(ws + L.yq);
     return s;
 }
 
 constexpr int kOverflow = -2;
 constexpr int kFallback = -3;
+
+/* Outcome of one column in one launch. */
+enum ColumnOutcome { kDone = 0, kMoveOn = 1, kRedoFp64 = 2 };
 
 /* Mixed path guards.  A selection decision whose margin is below kGuard (on
  * the scale of ||r||^2 for the scores, of epsilon for the stop test) is not
@@ -284,9 +308,10 @@ __device__ __forceinline__ void apply_reflector_f(const float* v, float* c, int 
  * with A(I,J) read in FP64 from the CSC columns and R promoted to FP64.
  * Returns 0, or kFallback when the FP32 factor is too ill-conditioned or the
  * corrections have not settled after kRefineSteps steps. */
+/* Householder QR of the FP32 copy of A(I,J) with the FP32 right-hand side
+ * carried along; kFallback when min |R_ii| <= kCondGuard max |R_ii|. */
 template <int G>
-__device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
-                                              int m, int n)
+__device__ __forceinline__ int qr_f32(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
@@ -321,7 +346,6 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
             apply_reflector_f<G>(ai, s.bq32, i, m, tau);
         }
     }
-    /* conditioning of the FP32 factor (also rejects zero and NaN pivots) */
     float dmax = 0.f, dmin = INFINITY;
     for (int i = lane; i < n; i += G) {
         float d = fabsf(q[i * ld + i]);
@@ -332,7 +356,18 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
         dmax = fmaxf(dmax, __shfl_xor(dmax, o, G));
         dmin = fminf(dmin, __shfl_xor(dmin, o, G));
     }
-    if (!(dmin > kCondGuard * dmax)) return kFallback;
+    return (dmin > kCondGuard * dmax) ? 0 : kFallback;
+}
+
+template <int G>
+__device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
+                                              int m, int n)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const int ld = L.capI;
+    if (qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
+    const float* q = s.aq32;
     /* x0 in FP32, column-oriented back substitution */
     for (int K = n - 1; K >= 0; --K) {
         float xk = s.bq32[K] / q[K * ld + K];
@@ -640,11 +675,585 @@ __device__ __forceinline__ void count_solve(const BuildArgs& a, const WaveScratc
     wk[7] += dn;
 }
 
-/* Outcome of one column in one launch. */
-enum ColumnOutcome { kDone = 0, kMoveOn = 1, kRedoFp64 = 2 };
+/* v <- (R^T R)^-1 v in FP64 for the upper-triangular R stored column-major
+ * in q (leading dimension ld; FP64 or FP32 entries promoted to FP64). */
+template <int G, typename RT>
+__device__ __forceinline__ void normal_solve(const RT* q, int ld, double* v, int n)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    for (int K = 0; K < n; ++K) {  // R^T y = v
+        double yk = v[K] / (double)q[K * ld + K];
+        __syncthreads();
+        if (lane == 0) v[K] = yk;
+        for (int i = K + 1 + lane; i < n; i += G) v[i] = __fma_rn(-yk, (double)q[i * ld + K], v[i]);
+        __syncthreads();
+    }
+    for (int K = n - 1; K >= 0; --K) {  // R d = y
+        double dk = v[K] / (double)q[K * ld + K];
+        __syncthreads();
+        if (lane == 0) v[K] = dk;
+        for (int i = lane; i < K; i += G) v[i] = __fma_rn(-dk, (double)q[K * ld + i], v[i]);
+        __syncthreads();
+    }
+}
+
+/* out[i] = sum_j C(I_i, J_j) w_j over I (FP64, from the CSC columns); when
+ * `sub` is set, out = base - C w instead. */
+template <int G>
+__device__ __forceinline__ void csc_apply(const BuildArgs& a, const WaveScratch& s, int m, int n, const double* w,
+                                          const double* base, double* out)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    for (int i = lane; i < m; i += G) out[i] = base ? base[i] : 0.0;
+    __syncthreads();
+    const double sign = base ? -1.0 : 1.0;
+    for (int jj = 0; jj < n; ++jj) {
+        double wj = sign * w[jj];
+        int j = s.J[jj];
+        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+        for (int e = lane; e < cnt; e += G) {
+            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+            out[pos] = __fma_rn(wj, a.A.val[e0 + e], out[pos]);
+        }
+        __syncthreads();
+    }
+}
+
+/* out_j = sum_i C(I_i, J_j) v_i (FP64, one group reduction per column). */
+template <int G>
+__device__ __forceinline__ void csc_apply_t(const BuildArgs& a, const WaveScratch& s, int m, int n, const double* v,
+                                            double* out)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    for (int jj = 0; jj < n; ++jj) {
+        int j = s.J[jj];
+        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+        double p = 0.0;
+        for (int e = lane; e < cnt; e += G) {
+            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+            p = __fma_rn(a.A.val[e0 + e], v[pos], p);
+        }
+        p = Gr::sum_fixed(p);
+        if (lane == 0) out[jj] = p;
+    }
+    __syncthreads();
+}
+
+/* Adjoint of one column of a build with fixed pattern J (M's column).  With
+ * m = argmin ||C(I,J) m - b(I)|| and g = dL/dm:
+ *   r = b - C m,  R^T R lambda = g,  s = C lambda,
+ *   dL/dC(i, j) = r_i lambda_j - s_i m_j   (accumulated into gC, atomics),
+ *   dL/db(i)    = s_i                      (written into gB).
+ * The QR is recomputed (FP64 in the fixed order, or FP32 with refinement of
+ * the normal-equation solve in mixed mode). */
+template <int G, bool kMixed>
+__device__ __forceinline__ int backward_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
+                                               int col)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const int ld = L.capI;
+    int p0 = a.p_ptr[col], n = a.p_ptr[col + 1] - p0;
+    if (n > L.capJ) return kMoveOn;
+    if (n == 0) return kDone;
+    for (int i = lane; i < n; i += G) {
+        s.J[i] = a.p_ind[p0 + i];
+        s.y[i] = a.m_in[p0 + i];
+        s.g[i] = a.gM[p0 + i];
+    }
+    __syncthreads();
+    int m = build_I<G>(a, L, s, n);
+    if (m == kOverflow) return kMoveOn;
+    if (m < n) return kDone;  // degenerate column: no gradient
+    gather_ahat<G, kMixed>(a, L, s, m, n);
+    make_rhs<G>(a, s, m, col);
+    if (kMixed) {
+        if (qr_f32<G>(L, s, m, n) == kFallback) return kRedoFp64;
+    } else {
+        if (ls_solve<G>(L, s, m, n) & (COL_SINGULAR | COL_UNDERDET)) return kDone;
+    }
+    /* r = b - C m (into s.r) */
+    csc_apply<G>(a, s, m, n, s.y, s.b0, s.r);
+    if (kMixed) {
+        for (int i = lane; i < n; i += G) s.z[i] = s.g[i];
+        __syncthreads();
+        normal_solve<G>(s.aq32, ld, s.g, n);
+        bool ok = false;
+        for (int it = 0; it < kRefineSteps; ++it) {
+            csc_apply<G>(a, s, m, n, s.g, nullptr, s.b0);  // u = C lambda
+            csc_apply_t<G>(a, s, m, n, s.b0, s.b);          // C^T u
+            for (int i = lane; i < n; i += G) s.b[i] = s.z[i] - s.b[i];
+            __syncthreads();
+            normal_solve<G>(s.aq32, ld, s.b, n);
+            double dmx = 0.0, lmx = 0.0;
+            for (int i = lane; i < n; i += G) {
+                double l = s.g[i] + s.b[i];
+                s.g[i] = l;
+                dmx = fmax(dmx, fabs(s.b[i]));
+                lmx = fmax(lmx, fabs(l));
+            }
+            for (int o = G / 2; o >= 1; o >>= 1) {
+                dmx = fmax(dmx, __shfl_xor(dmx, o, G));
+                lmx = fmax(lmx, __shfl_xor(lmx, o, G));
+            }
+            __syncthreads();
+            if (dmx <= 1e-10 * lmx) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) return kRedoFp64;
+    } else {
+        normal_solve<G>(s.aqr, ld, s.g, n);
+    }
+    /* s = C lambda (into s.b0) */
+    csc_apply<G>(a, s, m, n, s.g, nullptr, s.b0);
+    for (int jj = 0; jj < n; ++jj) {
+        double lj = s.g[jj], xj = s.y[jj];
+        int j = s.J[jj];
+        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+        for (int e = lane; e < cnt; e += G) {
+            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+            double v = __fma_rn(s.r[pos], lj, -s.b0[pos] * xj);
+            atomicAdd(&a.gC[e0 + e], v);
+        }
+    }
+    if (a.gB && a.B.colptr) {
+        int e0 = a.B.colptr[col], cnt = a.B.colptr[col + 1] - e0;
+        for (int e = lane; e < cnt; e += G) {
+            int row = a.B.rowind[e0 + e];
+            int pos = lower_bound_i(s.I, m, row);
+            a.gB[e0 + e] = (pos < m && s.I[pos] == row) ? s.b0[pos] : 0.0;
+        }
+    }
+    __syncthreads();
+    return kDone;
+}
 
 /* One column (Spai_Unrestrained<T>::SPAI_Column).  kMoveOn: it does not fit
  * this launch's layout; kRedoFp64: the mixed path declined a decision. */
+/* Householder vector of storage column c over storage rows [c, rows)
+ * (dlarfg in the fixed order).  Returns tau. */
+template <int G>
+__device__ __forceinline__ double make_reflector(double* ai, int c, int rows)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    double tau = 0.0;
+    if (c + 1 < rows) {
+        double alpha = ai[c];
+        double p = 0.0;
+        for (int r = lane; r < rows; r += G)
+            if (r > c) p = __fma_rn(ai[r], ai[r], p);
+        double xn2 = Gr::sum_fixed(p);
+        if (xn2 != 0.0) {
+            double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
+            double beta = -copysign(nrm, alpha);
+            tau = (beta - alpha) / beta;
+            double scal = 1.0 / (alpha - beta);
+            __syncthreads();
+            for (int r = lane; r < rows; r += G)
+                if (r > c) ai[r] = ai[r] * scal;
+            if (lane == 0) ai[c] = beta;
+            __syncthreads();
+        }
+    }
+    return tau;
+}
+
+/*
+ * Spai_QRUpdate<T>::SPAI_Column with the dense update (-qr 2,
+ * src/Spai_QRUpdate.imp:33-215, QRZerlegungLapack and qrupdate_voll in
+ * src/qrupdates.cpp:37-327).  The factor of A(I,J) is kept across the
+ * augmentation steps in the group's working set: rows are stored in the order
+ * they arrived (sorted I, then each step's new rows at the bottom), columns in
+ * the order they were added (new columns at the right).  A step applies the
+ * existing reflectors to the new columns over the rows each reflector spans,
+ * factors the block below the old R, re-applies every reflector to the stored
+ * right-hand side and back-substitutes row by row (sequential, unfused, as
+ * qrupdate_voll does); the solution is put back in ascending column order.
+ */
+template <int G>
+__device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
+                                                 int col)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const int ld = L.capI;
+    double* wk = a.work ? a.work + (size_t)col * kNumWork : nullptr;
+    int p0 = a.p_ptr[col], n = a.p_ptr[col + 1] - p0;
+    if (n > L.capJ) return kMoveOn;
+    if (wk && lane < kNumWork) wk[lane] = 0.0;
+    for (int i = lane; i < n; i += G) s.J[i] = a.p_ind[p0 + i];
+    __syncthreads();
+    int st = 0, solves = 0;
+    /* first factorization: QRZerlegungLapack */
+    int m = build_I<G>(a, L, s, n);
+    if (m == kOverflow) return kMoveOn;
+    st |= make_rhs<G>(a, s, m, col);
+    gather_ahat<G, false>(a, L, s, m, n);
+    for (int i = lane; i < m; i += G) {
+        s.srow[i] = s.I[i];
+        s.spos[i] = i;
+        s.bperm[i] = s.b[i];
+        s.yq[i] = s.b[i];
+    }
+    for (int c = lane; c < n; c += G) {
+        s.colid[c] = s.J[c];
+        s.reflrows[c] = m;
+        s.tau[c] = 0.0;
+    }
+    __syncthreads();
+    int k = m < n ? m : n;
+    for (int i = 0; i < k; ++i) {
+        double* ai = s.aqr + i * ld;
+        double t = make_reflector<G>(ai, i, m);
+        if (lane == 0) s.tau[i] = t;
+        if (t != 0.0)
+            for (int j = i + 1; j < n; ++j) apply_reflector<G>(ai, s.aqr + j * ld, i, m, t);
+        __syncthreads();
+    }
+    for (int i = 0; i < k; ++i) {
+        double t = s.tau[i];
+        if (t != 0.0) apply_reflector<G>(s.aqr + i * ld, s.yq, i, m, t);
+    }
+    __syncthreads();
+    if (m < n) {
+        for (int i = lane; i < n; i += G) s.b[i] = 0.0;
+        st |= COL_UNDERDET;
+        __syncthreads();
+    } else {
+        bool zero = false;
+        for (int i = lane; i < n; i += G) zero |= (s.aqr[i * ld + i] == 0.0);
+        if (Gr::ballot(zero) != 0ull) {
+            st |= COL_SINGULAR;
+            for (int i = lane; i < n; i += G) s.b[i] = s.yq[i];
+            __syncthreads();
+        } else {
+            for (int K = n - 1; K >= 0; --K) {
+                double xk = s.yq[K];
+                if (xk != 0.0) {
+                    xk = xk / s.aqr[K * ld + K];
+                    double nx = -xk;
+                    __syncthreads();
+                    if (lane == 0) s.yq[K] = xk;
+                    for (int i = lane; i < K; i += G) s.yq[i] = __fma_rn(nx, s.aqr[K * ld + i], s.yq[i]);
+                }
+                __syncthreads();
+            }
+            for (int i = lane; i < n; i += G) s.b[i] = s.yq[i];
+            __syncthreads();
+        }
+    }
+    ++solves;
+    count_solve<G>(a, s, m, n, wk);
+    double rn = residual_norm<G>(a, s, m, n);
+    int rows = m, cols = n;
+    for (int step = 0; step < a.ns && rn > a.epsilon; ++step) {
+        /* keep J, I and the storage positions of I */
+        for (int i = lane; i < n; i += G) s.jold[i] = s.J[i];
+        for (int i = lane; i < m; i += G) {
+            s.iold[i] = s.I[i];
+            s.sposold[i] = s.spos[i];
+        }
+        __syncthreads();
+        const int nold = n, mold = m;
+        int nn = augment<G, false>(a, L, s, col, m, n, rn, &st, wk);
+        if (nn == kOverflow) return kMoveOn;
+        if (nn < 0) break;
+        n = nn;
+        if (n > L.capJ) return kMoveOn;
+        m = build_I<G>(a, L, s, n);
+        if (m == kOverflow) return kMoveOn;
+        st |= make_rhs<G>(a, s, m, col);
+        /* storage positions: old rows keep theirs, new rows go to the bottom in
+         * ascending order; b_perm grows by e_col on the new rows */
+        int newr = 0;
+        for (int base = 0; base < m; base += G) {
+            int i = base + lane;
+            bool isnew = false;
+            int sp = 0;
+            if (i < m) {
+                int p = lower_bound_i(s.iold, mold, s.I[i]);
+                isnew = !(p < mold && s.iold[p] == s.I[i]);
+                if (!isnew) sp = s.sposold[p];
+            }
+            unsigned long long mask = Gr::ballot(isnew);
+            if (isnew) {
+                sp = rows + newr + __popcll(mask & Gr::lanes_below());
+                s.srow[sp] = s.I[i];
+                s.bperm[sp] = (s.I[i] == col) ? 1.0 : 0.0;
+            }
+            if (i < m) s.spos[i] = sp;
+            newr += __popcll(mask);
+        }
+        const int rows_new = rows + newr;
+        if (rows_new > L.capI) return kMoveOn;
+        /* old columns: zero on the new rows */
+        for (int c = 0; c < cols; ++c)
+            for (int r = rows + lane; r < rows_new; r += G) s.aqr[c * ld + r] = 0.0;
+        __syncthreads();
+        /* new columns J \ J_old in ascending order, at the right */
+        int newc = 0;
+        for (int t = 0; t < n; ++t) {
+            int j = s.J[t];
+            int p = lower_bound_i(s.jold, nold, j);
+            if (p < nold && s.jold[p] == j) continue;
+            int c = cols + newc;
+            ++newc;
+            double* cc = s.aqr + c * ld;
+            for (int r = lane; r < rows_new; r += G) cc[r] = 0.0;
+            __syncthreads();
+            int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+            for (int e = lane; e < cnt; e += G) {
+                int q = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+                cc[s.spos[q]] = a.A.val[e0 + e];
+            }
+            if (lane == 0) {
+                s.colid[c] = j;
+                s.reflrows[c] = rows_new;
+                s.tau[c] = 0.0;
+            }
+            __syncthreads();
+            for (int q = 0; q < cols; ++q) {
+                double t = s.tau[q];
+                if (t != 0.0) apply_reflector<G>(s.aqr + q * ld, cc, q, s.reflrows[q], t);
+            }
+        }
+        const int cols_new = cols + newc;
+        /* factor the block below the old R (dgeqrf on B2) */
+        for (int c = cols; c < cols_new; ++c) {
+            if (c >= rows_new) break;
+            double* ac = s.aqr + c * ld;
+            double t = make_reflector<G>(ac, c, rows_new);
+            if (lane == 0) s.tau[c] = t;
+            if (t != 0.0)
+                for (int u = c + 1; u < cols_new; ++u) apply_reflector<G>(ac, s.aqr + u * ld, c, rows_new, t);
+            __syncthreads();
+        }
+        rows = rows_new;
+        cols = cols_new;
+        /* Q^T b from the stored right-hand side */
+        for (int r = lane; r < rows; r += G) s.yq[r] = s.bperm[r];
+        __syncthreads();
+        for (int c = 0; c < cols; ++c) {
+            double t = s.tau[c];
+            if (t != 0.0) apply_reflector<G>(s.aqr + c * ld, s.yq, c, s.reflrows[c], t);
+        }
+        __syncthreads();
+        if (rows < cols) {
+            st |= COL_UNDERDET;
+            for (int i = lane; i < cols; i += G) s.yq[i] = 0.0;
+            __syncthreads();
+        } else if (lane == 0) {
+            /* row-oriented back substitution (src/qrupdates.cpp:306-314) */
+            for (int i = cols - 1; i >= 0; --i) {
+                double z = 0.0;
+                for (int j = i + 1; j < cols; ++j) z = __dadd_rn(z, __dmul_rn(s.aqr[j * ld + i], s.yq[j]));
+                double d = s.aqr[i * ld + i];
+                if (d == 0.0) st |= COL_SINGULAR;
+                s.yq[i] = __ddiv_rn(__dsub_rn(s.yq[i], z), d);
+            }
+        }
+        st = Gr::bcast0(st);
+        __syncthreads();
+        /* solution in ascending column order */
+        for (int t = lane; t < n; t += G) {
+            int j = s.J[t];
+            int c = 0;
+            while (s.colid[c] != j) ++c;
+            s.b[t] = s.yq[c];
+        }
+        __syncthreads();
+        ++solves;
+        count_solve<G>(a, s, m, n, wk);
+        rn = residual_norm<G>(a, s, m, n);
+        if (step == a.ns - 1) break;
+    }
+    size_t o = (size_t)col * a.cap;
+    for (int i = lane; i < n; i += G) {
+        a.m_idx[o + i] = s.J[i];
+        a.m_val[o + i] = s.b[i];
+    }
+    if (lane == 0) {
+        a.m_len[col] = n;
+        a.res[col] = rn;
+        a.nsolve[col] = solves;
+        a.status[col] = st | a.status_or;
+    }
+    __syncthreads();
+    return kDone;
+}
+
+/*
+ * Device pattern->R cache (Spai_Caching, src/Spai_Caching.imp:45-332).
+ * Key: the reference's value hash of A(I,J) (Cache<double>::Compute_Key,
+ * src/Cache.cpp:42-57: ret = 31 ret + mix + lo32 ^ hi32 over the values in
+ * column-major order), evaluated as the equivalent polynomial in parallel.
+ * A slot holds (m, n, key, state) and, in a.cache_data, A(I,J), its
+ * Householder factor, tau, the right-hand side, the solution and the solve's
+ * status.  A hit needs an equal key and an exactly equal A(I,J) (the
+ * Compare_A_Hat confirmation); with an exactly equal right-hand side the
+ * stored solution is reused, otherwise the stored factor is applied to the new
+ * right-hand side.  Either way the result is bitwise the uncached one.  Slots
+ * are claimed with a compare-and-swap (0 empty -> 1 writing) and published
+ * after a release fence (-> 2 ready); they are never overwritten within a
+ * build, so readers need no lock.
+ */
+constexpr int kCacheProbes = 8;
+
+__device__ __forceinline__ uint32_t hash_double(double x)
+{
+    unsigned long long b = __double_as_longlong(x);
+    return (uint32_t)b ^ (uint32_t)(b >> 32);
+}
+
+__device__ __forceinline__ uint32_t pow31(uint32_t e)
+{
+    uint32_t r = 1u, b = 31u;
+    while (e) {
+        if (e & 1u) r *= b;
+        b *= b;
+        e >>= 1;
+    }
+    return r;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t cache_key(const double* A, int ld, int m, int n)
+{
+    using Gr = Group<G>;
+    const uint32_t mix = 0x3FC0A3D7u ^ 0x0A3D70A4u;  // lo32 ^ hi32 of 0.13
+    const uint32_t N = (uint32_t)(m * n);
+    uint32_t acc = 0u;
+    for (int t = Gr::lane(); t < m * n; t += G) {
+        int j = t / m, i = t - j * m;
+        acc += (mix + hash_double(A[j * ld + i])) * pow31(N - 1u - (uint32_t)t);
+    }
+    for (int o = G / 2; o >= 1; o >>= 1) acc += (uint32_t)__shfl_xor((int)acc, o, G);
+    return acc;
+}
+
+/* doubles per slot: A, factor (m x n each), tau, x (n), b (m) */
+__device__ __forceinline__ size_t cache_slot_doubles(const BuildArgs& a)
+{
+    return 2 * (size_t)a.cache_capI * a.cache_capJ + 2 * (size_t)a.cache_capJ + (size_t)a.cache_capI;
+}
+
+/* Least squares with the cache.  On entry s.aqr holds A(I,J) and s.b the
+ * right-hand side; on exit as ls_solve. */
+template <int G>
+__device__ __forceinline__ int ls_solve_cached(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
+                                               int m, int n)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const int ld = L.capI;
+    if (m > a.cache_capI || n > a.cache_capJ || m == 0 || n == 0) return ls_solve<G>(L, s, m, n);
+    const uint32_t key = cache_key<G>(s.aqr, ld, m, n);
+    const size_t sd = cache_slot_doubles(a);
+    const int mask = a.cache_slots - 1;
+    int slot = (int)(key & (uint32_t)mask);
+    int claimed = -1;
+    for (int p = 0; p < kCacheProbes; ++p, slot = (slot + 1) & mask) {
+        int state = __hip_atomic_load(&a.cache_state[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (state == 0) {
+            int won = 0;
+            if (lane == 0) won = atomicCAS(&a.cache_state[slot], 0, 1) == 0;
+            won = Gr::bcast0(won);
+            if (won) {
+                claimed = slot;
+                break;
+            }
+            continue;  // another group took it
+        }
+        if (state != 2 || a.cache_key[slot] != key || a.cache_m[slot] != m || a.cache_n[slot] != n) continue;
+        const double* d = a.cache_data + (size_t)slot * sd;
+        bool diff = false;
+        for (int t = lane; t < m * n; t += G) {
+            int j = t / m, i = t - j * m;
+            diff |= __double_as_longlong(d[t]) != __double_as_longlong(s.aqr[j * ld + i]);
+        }
+        if (Gr::ballot(diff) != 0ull) continue;
+        /* hit */
+        const double* fac = d + (size_t)a.cache_capI * a.cache_capJ;
+        const double* tau = fac + (size_t)a.cache_capI * a.cache_capJ;
+        const double* x = tau + a.cache_capJ;
+        const double* b = x + a.cache_capJ;
+        bool bdiff = false;
+        for (int i = lane; i < m; i += G) bdiff |= __double_as_longlong(b[i]) != __double_as_longlong(s.b[i]);
+        const int st_solve = a.cache_status[slot];
+        if (lane == 0) atomicAdd(a.cache_hits, 1);
+        if (Gr::ballot(bdiff) == 0ull) {
+            for (int i = lane; i < n; i += G) s.b[i] = x[i];
+            __syncthreads();
+            return st_solve;
+        }
+        for (int t = lane; t < m * n; t += G) {
+            int j = t / m, i = t - j * m;
+            s.aqr[j * ld + i] = fac[t];
+        }
+        __syncthreads();
+        int k = m < n ? m : n;
+        for (int i = 0; i < k; ++i)
+            if (tau[i] != 0.0) apply_reflector<G>(s.aqr + i * ld, s.b, i, m, tau[i]);
+        __syncthreads();
+        if (m < n) {
+            for (int i = lane; i < n; i += G) s.b[i] = 0.0;
+            __syncthreads();
+            return COL_UNDERDET;
+        }
+        if (st_solve & COL_SINGULAR) return COL_SINGULAR;
+        for (int K = n - 1; K >= 0; --K) {
+            double xk = s.b[K];
+            if (xk != 0.0) {
+                xk = xk / s.aqr[K * ld + K];
+                double nx = -xk;
+                __syncthreads();
+                if (lane == 0) s.b[K] = xk;
+                for (int i = lane; i < K; i += G) s.b[i] = __fma_rn(nx, s.aqr[K * ld + i], s.b[i]);
+            }
+            __syncthreads();
+        }
+        return 0;
+    }
+    if (claimed < 0) return ls_solve<G>(L, s, m, n);
+    /* miss with a claimed slot: store A and b, solve, store the factor */
+    double* d = a.cache_data + (size_t)claimed * sd;
+    double* fac = d + (size_t)a.cache_capI * a.cache_capJ;
+    double* tau = fac + (size_t)a.cache_capI * a.cache_capJ;
+    double* x = tau + a.cache_capJ;
+    double* b = x + a.cache_capJ;
+    for (int t = lane; t < m * n; t += G) {
+        int j = t / m, i = t - j * m;
+        d[t] = s.aqr[j * ld + i];
+    }
+    for (int i = lane; i < m; i += G) b[i] = s.b[i];
+    __syncthreads();
+    int st_solve = ls_solve<G>(L, s, m, n);
+    for (int t = lane; t < m * n; t += G) {
+        int j = t / m, i = t - j * m;
+        fac[t] = s.aqr[j * ld + i];
+    }
+    for (int i = lane; i < n; i += G) {
+        tau[i] = s.tau[i];
+        x[i] = s.b[i];
+    }
+    if (lane == 0) {
+        a.cache_key[claimed] = key;
+        a.cache_m[claimed] = m;
+        a.cache_n[claimed] = n;
+        a.cache_status[claimed] = st_solve;
+    }
+    __threadfence();
+    __syncthreads();
+    if (lane == 0) __hip_atomic_store(&a.cache_state[claimed], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return st_solve;
+}
+
 template <int G, bool kMixed>
 __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int col)
@@ -666,6 +1275,8 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         st |= make_rhs<G>(a, s, m, col);
         if (kMixed) {
             if (ls_solve_mixed<G>(a, L, s, m, n) == kFallback) return kRedoFp64;
+        } else if (a.cache_slots > 0) {
+            st |= ls_solve_cached<G>(a, L, s, m, n);
         } else {
             st |= ls_solve<G>(L, s, m, n);
         }
@@ -707,7 +1318,9 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
     if (flag) out[base + __popcll(mask & ((1ull << lane_id()) - 1ull))] = col;
 }
 
-template <int G, bool kLds, bool kMixed>
+/* kMode: 0 FP64 build, 1 mixed build, 2 FP64 adjoint, 3 mixed adjoint,
+ * 4 FP64 build with the dense QR update (-qr 2). */
+template <int G, bool kLds, int kMode>
 __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
 {
@@ -731,12 +1344,17 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
             int outcome = kDone;
             if (q < nq) {
                 col = queue[q];
-                outcome = solve_column<G, kMixed>(a, L, s, col);
+                if (kMode < 2)
+                    outcome = solve_column<G, kMode == 1>(a, L, s, col);
+                else if (kMode < 4)
+                    outcome = backward_column<G, kMode == 3>(a, L, s, col);
+                else
+                    outcome = solve_column_qrup<G>(a, L, s, col);
             }
             __syncthreads();
             const bool leader = Group<G>::lane() == 0;
             wave_append(leader && outcome == kMoveOn, col, a.overflow, a.n_overflow);
-            if (kMixed) wave_append(leader && outcome == kRedoFp64, col, a.fallback, a.n_fallback);
+            if (kMode == 1 || kMode == 3) wave_append(leader && outcome == kRedoFp64, col, a.fallback, a.n_fallback);
         }
     }
 }
@@ -775,28 +1393,30 @@ __global__ void copy_ints_kernel(const int* src, int* dst, int count)
     if (i < count) dst[i] = src[i];
 }
 
-void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, bool mixed, const int* queue,
+void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mode, const int* queue,
                    const int* nqueue, int* head, int nwaves, hipStream_t stream)
 {
     size_t lds = L.per_wave * (kWave / group);
-#define MSPAI_LAUNCH(G, MX)                                                                                    \
-    hipLaunchKernelGGL((spai_column_kernel<G, true, MX>), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue, \
+#define MSPAI_LAUNCH(G, MODE)                                                                                    \
+    hipLaunchKernelGGL((spai_column_kernel<G, true, MODE>), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue, \
                        nqueue, head)
-    if (mixed) {
-        if (group == 16)
-            MSPAI_LAUNCH(16, true);
-        else if (group == 32)
-            MSPAI_LAUNCH(32, true);
-        else
-            MSPAI_LAUNCH(64, true);
-    } else {
-        if (group == 16)
-            MSPAI_LAUNCH(16, false);
-        else if (group == 32)
-            MSPAI_LAUNCH(32, false);
-        else
-            MSPAI_LAUNCH(64, false);
+#define MSPAI_LAUNCH_G(MODE)      \
+    do {                          \
+        if (group == 16)          \
+            MSPAI_LAUNCH(16, MODE); \
+        else if (group == 32)     \
+            MSPAI_LAUNCH(32, MODE); \
+        else                      \
+            MSPAI_LAUNCH(64, MODE); \
+    } while (0)
+    switch (mode) {
+    case 0: MSPAI_LAUNCH_G(0); break;
+    case 1: MSPAI_LAUNCH_G(1); break;
+    case 2: MSPAI_LAUNCH_G(2); break;
+    case 3: MSPAI_LAUNCH_G(3); break;
+    default: MSPAI_LAUNCH_G(4); break;
     }
+#undef MSPAI_LAUNCH_G
 #undef MSPAI_LAUNCH
 }
 
@@ -812,11 +1432,18 @@ void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream)
     hipLaunchKernelGGL(copy_ints_kernel, dim3(1), dim3(64), 0, stream, src, dst, count);
 }
 
-void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, const int* queue, const int* nqueue,
-                         int* head, int nwaves, hipStream_t stream)
+void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, int mode, const int* queue,
+                         const int* nqueue, int* head, int nwaves, hipStream_t stream)
 {
-    hipLaunchKernelGGL((spai_column_kernel<64, false, false>), dim3(nwaves), dim3(kWave), 0, stream, a, L,
-                       queue, nqueue, head);
+    if (mode == 2)
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 2>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+                           nqueue, head);
+    else if (mode == 4)
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 4>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+                           nqueue, head);
+    else
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+                           nqueue, head);
 }
 
 }  // namespace mspai

@@ -12,9 +12,11 @@ namespace mspai {
 
 /* Column kernel with the working set in LDS: one column per group of
  * `group` lanes (16, 32 or 64), L.per_wave bytes of LDS per group; columns
- * that do not fit go to a.overflow.  mixed: FP32 QR + FP64 refinement, and
- * columns whose selection margins are too small go to a.fallback. */
-void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, bool mixed, const int* queue,
+ * that do not fit go to a.overflow.  mode 0: FP64 build; 1: FP32 QR + FP64
+ * refinement, columns whose selection margins are too small go to
+ * a.fallback; 2/3: the adjoint in FP64 / mixed; 4: FP64 build with the
+ * dense QR update (-qr 2). */
+void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mode, const int* queue,
                    const int* nqueue, int* head, int nwaves, hipStream_t stream);
 
 /* Route columns to bin queues: queue b < nbins at queues + b*n with its
@@ -25,9 +27,10 @@ void launch_classify(const BuildArgs& a, int nbins, const BinEdges& edges, int* 
 /* dst[0..count) = src[0..count) on the stream (count <= 64). */
 void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream);
 
-/* Same kernel with the working set in HBM (a.scratch, L.per_wave per wave). */
-void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, const int* queue, const int* nqueue,
-                         int* head, int nwaves, hipStream_t stream);
+/* FP64 kernel (mode 0 build, 2 adjoint, 4 QR-update build) with the working
+ * set in HBM (a.scratch, L.per_wave per wave). */
+void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, int mode, const int* queue,
+                         const int* nqueue, int* head, int nwaves, hipStream_t stream);
 
 }  // namespace mspai
 

@@ -163,3 +163,44 @@ def test_full_c5_sampled_columns(oracle, gpu):
         J = M.indices[M.indptr[k]:M.indptr[k + 1]]
         I, Ah, bb, x = column_lstsq(prob.A, J, k)
         assert ls_close(Ah, bb, M.data[M.indptr[k]:M.indptr[k + 1]], x)
+
+
+QR_CASES = {
+    "c2_convdiff_64": lambda: gen.config2(64),
+    "c2_convdiff_256": lambda: gen.config2(256),
+    "c3_random_16384": lambda: gen.config3(n=16384, bw=800),
+    "cs_fs_183_1": lambda: csparse_problem("fs_183_1", eps=0.2, mn=5, ns=6),
+    "cs_bcsstk01": lambda: csparse_problem("bcsstk01"),
+    "cs_west0067": lambda: csparse_problem("west0067"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(QR_CASES))
+def test_qr_update_bitwise(oracle, gpu, name):
+    """-qr 2: the factor kept across steps (qrupdate_voll) on the GPU equals the
+    oracle's restatement bit for bit."""
+    prob = QR_CASES[name]()
+    print(f"[qr2 {name}] n={prob.n}", flush=True)
+    ref, M, info, _ = run_both(oracle, gpu, prob, qr=2)
+    assert_bitwise(oracle, prob, ref, M, info)
+
+
+def test_qr_update_full_c3(oracle, gpu):
+    prob = gen.config3()
+    print(f"[qr2 full {prob.name}] n={prob.n}", flush=True)
+    ref, M, info, _ = run_both(oracle, gpu, prob, qr=2)
+    assert_bitwise(oracle, prob, ref, M, info)
+
+
+@pytest.mark.parametrize("make", [lambda: gen.config1(64), lambda: gen.config2(256), lambda: gen.config4(16),
+                                  lambda: gen.config3(n=16384, bw=800), lambda: csparse_problem("fs_183_1")])
+def test_cache_is_bitwise_neutral(oracle, gpu, make):
+    """-cs N: cached factors/solutions reproduce the uncached build bit for bit,
+    and the structured grids hit the cache."""
+    prob = make()
+    ref = oracle.spai(prob, threads=THREADS)
+    M, info, stats = gpu.spai(prob, cache=4096)
+    assert_bitwise(oracle, prob, ref, M, info)
+    print(f"[cache {prob.name}] hits {stats['n_cache_hits']} of {int(info['nsolve'].sum())} solves", flush=True)
+    if prob.name in ("laplace2d_static", "convdiff2d_adaptive", "laplace3d_probing"):
+        assert stats["n_cache_hits"] > 0.5 * info["nsolve"].sum()

@@ -49,6 +49,33 @@ def _fro_residual(A, M):
     return float(np.sqrt((R.multiply(R)).sum()))
 
 
+def _bwd_fwd(mspai_amd, gen, torch):
+    """Backward/forward device time of the adjoint on the MSPAI probing
+    workload (3D Laplacian 64^3, static pattern(A), rho = 100)."""
+    prob = gen.config4()
+    b = mspai_amd.Builder(0)
+    b.upload(prob.A, prob.row_pattern())
+    b.set_pattern(prob.P)
+    b.set_target(prob.B)
+    out = {"workload": "MSPAI probing, 3D Laplacian 64^3 (n=262144), static P(A), rho=100"}
+    for name, precision in (("fp64", 0), ("mixed", 1)):
+        fw, bw = [], []
+        for it in range(4):
+            st = b.build(max_impr_steps=0, precision=precision)
+            gm = torch.ones(st["nnz"], dtype=torch.float64, device="cuda")
+            gc = torch.empty(prob.A.nnz, dtype=torch.float64, device="cuda")
+            gb = torch.empty(prob.B.nnz, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            sb = b.backward_device(gm.data_ptr(), gc.data_ptr(), gb.data_ptr(), precision=precision)
+            if it:
+                fw.append(st["build_ms"])
+                bw.append(sb["build_ms"])
+        out[name] = {"fwd_ms": float(np.mean(fw)), "bwd_ms": float(np.mean(bw)),
+                     "bwd_over_fwd": float(np.mean(bw) / np.mean(fw))}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +105,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    try:  # torch first: one HIP runtime for torch and the library
+        import torch
+    except Exception:  # pragma: no cover
+        torch = None
     import mspai_amd
     from mspai_amd import gen
 
@@ -88,9 +119,11 @@ def main():
     b.set_pattern(prob.P)
     base_opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
                      use_mean=prob.use_mean)
+    # one-time set-up outside the timed region: choose the size bins
+    tune = b.autotune(repeats=2, **base_opts)
 
     def timed(precision):
-        opts = dict(base_opts, precision=precision)
+        opts = dict(base_opts, precision=precision, binning=2)
         for _ in range(args.warmup):
             b.build(**opts)
         builds, solves = [], []
@@ -106,7 +139,20 @@ def main():
 
     r64 = timed(0)
     rmx = timed(1) if args.precision in ("mixed", "both") else None
+    # the reference's LRU cache option (-cs): same bits, repeated A(I,J) blocks reuse factors
+    cache_run = None
+    if args.precision == "both":
+        opts_c = dict(base_opts, precision=0, binning=2, cache=65536)
+        for _ in range(args.warmup):
+            b.build(**opts_c)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            stc = b.build(**opts_c)
+        cache_run = {"columns_per_s": n / ((time.perf_counter() - t0) / args.steps),
+                     "cache_hits_per_build": stc["n_cache_hits"],
+                     "pattern_equal_fp64": bool(np.array_equal(b.download().indices, r64["M"].indices))}
     b.close()
+    bwd = _bwd_fwd(mspai_amd, gen, torch) if (torch is not None and args.precision == "both") else None
     main_run = rmx if rmx is not None else r64
     M = main_run["M"]
     ms_per_step = main_run["ms_per_step"]
@@ -186,6 +232,9 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "qr_flop_per_launch": float(work[0] + work[1])},
         "bins": main_run["bins"],
+        "autotune": tune,
+        "fp64_with_cache": cache_run,
+        "bwd_fwd": bwd,
         "cpu_baseline": cpu,
         "parity": parity,
     }

@@ -51,7 +51,8 @@ typedef struct {
     int qr;                /* -qr  0 = fresh QR per step, 2 = dense QR update (0) */
     int cache;             /* -cs  device pattern->R cache entries, 0 = off (0) */
     int precision;         /* GPU: 0 = FP64, 1 = FP32 QR + FP64 refinement (0) */
-    int binning;           /* GPU: 1 = size-binned kernels, 0 = one generic bin (1) */
+    int binning;           /* GPU: 1 = default size bins, 2 = autotuned bins (mspai_autotune),
+                              0 = one generic bin (1) */
     int verbose;           /* -vb */
 } mspai_opts;
 
@@ -139,6 +140,22 @@ int mspai_result_values_device(mspai_ctx* ctx, double* d_out);
  */
 int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const double* d_gM, double* d_gC, double* d_gB,
                    mspai_stats* stats);
+
+/*
+ * Autotuner of the size bins (no counterpart in the reference): builds the
+ * uploaded problem with each candidate bin table (lane-group widths, |J| and
+ * |I| edges, resident-wave caps), `repeats` timed runs each after one warm-up,
+ * and keeps the fastest for later builds with opts->binning == 2.  Results do
+ * not depend on the table.  A new mspai_upload_csc clears the choice.
+ */
+#define MSPAI_MAX_TUNE 8
+typedef struct {
+    int ncandidates;
+    int chosen;
+    char chosen_name[32];
+    double candidate_ms[MSPAI_MAX_TUNE];  /* best column-kernel time per table */
+} mspai_tune_report;
+int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeats, mspai_tune_report* report);
 
 /* Sums over all columns of the 8 work counters of the last build:
  * QR flop, other solve flop (Q^T b, triangular solve, residual), gather

@@ -24,7 +24,7 @@ EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspa
             "mspai_build", "mspai_result_nnz", "mspai_download_csc",
             "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts",
             "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
-            "mspai_backward"]
+            "mspai_backward", "mspai_autotune"]
 
 
 class Opts(C.Structure):
@@ -37,6 +37,11 @@ class Stats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("solve_ms", C.c_double), ("nnz", C.c_longlong),
                 ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int),
                 ("n_cache_hits", C.c_int)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("ncandidates", C.c_int), ("chosen", C.c_int), ("chosen_name", C.c_char * 32),
+                ("candidate_ms", C.c_double * 8)]
 
 
 _lib = None
@@ -168,6 +173,16 @@ class Builder:
         nnz = C.c_longlong()
         self._chk(self.lib.mspai_result_nnz(self.h, C.byref(nnz)), "mspai_result_nnz")
         return nnz.value
+
+    def autotune(self, repeats=2, **build_opts):
+        kw = dict(epsilon=0.4, maxnew=5, max_impr_steps=5, use_mean=1, qr=0, cache=0, precision=0)
+        kw.update(build_opts)
+        o = Opts(kw["epsilon"], kw["maxnew"], kw["max_impr_steps"], kw["use_mean"], kw["qr"], kw["cache"],
+                 kw["precision"], 2, 0)
+        r = TuneReport()
+        self._chk(self.lib.mspai_autotune(self.h, C.byref(o), repeats, C.byref(r)), "mspai_autotune")
+        return {"chosen": r.chosen, "name": r.chosen_name.decode(),
+                "candidate_ms": list(r.candidate_ms[:r.ncandidates])}
 
     def work_totals(self):
         w = np.zeros(NWORK, np.float64)

@@ -119,6 +119,8 @@ struct mspai_ctx {
     double* d_wsum = nullptr;  // work-counter totals
     char* d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    char* d_tie = nullptr;
+    size_t tie_bytes = 0;
     void* d_cub = nullptr;
     size_t cub_bytes = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -130,6 +132,8 @@ struct mspai_ctx {
     int* d_cache_n = nullptr;
     int* d_cache_status = nullptr;
     double* d_cache_data = nullptr;
+    int tuned_table = -1;  // autotuner's choice for this matrix (binning == 2)
+    int last_table = 0;
     int last_nbins = 0;
     bool last_mixed = false;
     int last_binning = 1;
@@ -237,6 +241,7 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_counter);
     dfree(ctx->d_wsum);
     dfree(ctx->d_scratch);
+    dfree(ctx->d_tie);
     dfree(ctx->d_cache_state);
     dfree(ctx->d_cache_key);
     dfree(ctx->d_cache_m);
@@ -306,6 +311,7 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     ctx->maxcolnnz = maxc;
     ctx->maxrownnz = maxr;
     ctx->nnzM = -1;
+    ctx->tuned_table = -1;
     if (!ctx->pattern_set) {
         dfree(ctx->d_p_ptr);
         dfree(ctx->d_p_ind);
@@ -461,8 +467,15 @@ static size_t next_pow2(size_t x)
     return p;
 }
 
-/* Byte offsets of one per-wave working set holding the given sizes. */
-static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capSort, int maxnew, size_t capPairs)
+/* Buffers a launch needs besides the FP64 build's (bit flags). */
+enum LayoutNeeds { kNeedMixed = 1, kNeedBackward = 2, kNeedQrUpdate = 4 };
+
+/* Byte offsets of one group's working set holding the given sizes.  The
+ * std::sort tie-path buffers (pairs, stack) sit in the main region when
+ * tie_in_main, else in a separate per-group HBM region of tie_per_group
+ * bytes (LDS bins: the path is rare). */
+static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capSort, int maxnew, size_t capPairs,
+                                 int needs, bool tie_in_main)
 {
     ScratchLayout L{};
     capI = std::max<size_t>(capI, 1);
@@ -490,22 +503,37 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.tau = take((size_t)capJ * 8);
     L.jt = take(capU * 4);
     L.rho = take(capU * 8);
-    L.pairs = take((size_t)L.capPairs * sizeof(RhoPair));
     L.sel = take((size_t)std::max(1, maxnew) * 4);
-    L.stack = take(3 * introsort_detail::kStackFrames * sizeof(int));
-    L.bq32 = take(capB * 4);
-    L.g = take((size_t)capJ * 8);
-    L.y = take((size_t)capJ * 8);
-    L.z = take((size_t)capJ * 8);
-    L.srow = take(capB * 4);
-    L.spos = take(capB * 4);
-    L.iold = take(capB * 4);
-    L.sposold = take(capB * 4);
-    L.jold = take((size_t)capJ * 4);
-    L.colid = take((size_t)capJ * 4);
-    L.reflrows = take((size_t)capJ * 4);
-    L.bperm = take(capB * 8);
-    L.yq = take(capB * 8);
+    if (needs & (kNeedMixed | kNeedBackward)) {
+        L.bq32 = take(capB * 4);
+        L.g = take((size_t)capJ * 8);
+        L.y = take((size_t)capJ * 8);
+        L.z = take((size_t)capJ * 8);
+    }
+    if (needs & kNeedQrUpdate) {
+        L.srow = take(capB * 4);
+        L.spos = take(capB * 4);
+        L.iold = take(capB * 4);
+        L.sposold = take(capB * 4);
+        L.jold = take((size_t)capJ * 4);
+        L.colid = take((size_t)capJ * 4);
+        L.reflrows = take((size_t)capJ * 4);
+        L.bperm = take(capB * 8);
+        L.yq = take(capB * 8);
+    }
+    size_t tie_pairs = (size_t)L.capPairs * sizeof(RhoPair);
+    size_t tie_stack = 3 * introsort_detail::kStackFrames * sizeof(int);
+    if (tie_in_main) {
+        L.pairs = take(tie_pairs);
+        L.stack = take(tie_stack);
+        L.tie_in_main = 1;
+        L.tie_per_group = 0;
+    } else {
+        L.pairs = 0;
+        L.stack = align_up(tie_pairs, 16);
+        L.tie_in_main = 0;
+        L.tie_per_group = align_up(L.stack + tie_stack, 256);
+    }
     L.per_wave = align_up(o, 256);
     return L;
 }
@@ -513,43 +541,70 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
 /* Worst-case layout of the generic kernel: |J| <= cap,
  * |I| <= min(nrows, cap * max nnz per column), raw candidate list
  * <= (|I| + 1) * max nnz per row. */
-static ScratchLayout generic_layout(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive)
+static ScratchLayout generic_layout(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive, int needs)
 {
     size_t rawI = (size_t)cap * std::max(1, ctx->maxcolnnz);
     size_t capI = std::max<size_t>(std::min<size_t>(ctx->nrows, rawI), 1);
     size_t rawU = adaptive ? (capI + 1) * std::max(1, ctx->maxrownnz) : 1;
     size_t capU = adaptive ? std::min<size_t>(ctx->ncols, rawU) : 1;
-    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU);
+    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU, needs, true);
 }
 
-/* One size bin: a lane-group width and the LDS layout of one group. */
+/* One size bin: a lane-group width, a resident-wave cap and the LDS layout
+ * of one group. */
 struct Bin {
     int group;
+    int wcap;
     ScratchLayout L;
 };
 
-/* LDS layouts of the size bins, smallest first: columns that outgrow a bin
+/* A size-bin table: per bin the lane-group width, the |J|, |I|, distinct
+ * candidate, raw-list and tie-path capacities, and a cap on resident waves
+ * per CU (0 = as many as LDS allows, at most 32). */
+struct BinEdge {
+    int group, J, I, U, sort, pairs, wcap;
+};
+struct BinTable {
+    const char* name;
+    int count;
+    BinEdge e[4];
+};
+
+/* Candidate tables of the autotuner; table 0 is the default. */
+static const BinTable kTables[] = {
+    {"g16-g32-g64-g64", 4,
+     {{16, 8, 16, 64, 128, 64, 0}, {32, 16, 32, 128, 256, 64, 0}, {64, 32, 64, 256, 512, 128, 0},
+      {64, 40, 128, 512, 1024, 128, 0}}},
+    {"g32-g64-g64", 3,
+     {{32, 16, 32, 128, 256, 64, 0}, {64, 32, 64, 256, 512, 128, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
+    {"g16wide-g64-g64", 3,
+     {{16, 16, 16, 128, 256, 64, 0}, {64, 32, 64, 256, 512, 128, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
+    {"g64-g64", 2, {{64, 32, 64, 256, 512, 128, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
+    {"g16-g64", 2, {{16, 8, 16, 64, 128, 64, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
+    {"g16-g32-g64-g64/occ12", 4,
+     {{16, 8, 16, 64, 128, 64, 12}, {32, 16, 32, 128, 256, 64, 12}, {64, 32, 64, 256, 512, 128, 8},
+      {64, 40, 128, 512, 1024, 128, 4}}},
+};
+constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
+
+/* LDS layouts of a table's bins, smallest first: columns that outgrow a bin
  * move to the next one; what outgrows the last runs generic.  A bin that
  * already holds every column (by the worst-case bounds) ends the list. */
 static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive,
-                                    const ScratchLayout& G)
+                                    const ScratchLayout& G, const BinTable& table, int needs)
 {
-    struct Edge {
-        int group, J, I, U, sort, pairs;
-    };
-    const Edge edges[4] = {{16, 8, 16, 64, 128, 64},
-                           {32, 16, 32, 128, 256, 64},
-                           {64, 32, 64, 256, 512, 128},
-                           {64, 40, 128, 512, 1024, 128}};
+    (void)ctx;
     std::vector<Bin> out;
-    for (const Edge& e : edges) {
+    for (int t = 0; t < table.count; ++t) {
+        const BinEdge& e = table.e[t];
         int capJ = std::min(cap, e.J);
         size_t capI = std::min<size_t>(G.capI, e.I);
         size_t capU = adaptive ? std::min<size_t>(G.capU, e.U) : 1;
         size_t capSort = std::min<size_t>(G.capSort, e.sort);
-        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, std::min<size_t>(capU, e.pairs));
+        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, std::min<size_t>(capU, e.pairs), needs,
+                                      false);
         if (L.per_wave * (64 / e.group) > 64 * 1024) continue;
-        out.push_back({e.group, L});
+        out.push_back({e.group, e.wcap, L});
         if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU) &&
             L.capPairs >= G.capPairs)
             break;
@@ -564,6 +619,16 @@ static int ensure_scratch(mspai_ctx* ctx, size_t need)
     HIPCHK(hipMalloc((void**)&ctx->d_scratch, need));
     ctx->scratch_bytes = need;
     return MSPAI_OK;
+}
+
+/* Resident waves per CU for a bin: what 160 KB of LDS allows, at most 32,
+ * and at most the bin's cap. */
+static int waves_per_cu(const Bin& b)
+{
+    size_t lds = b.L.per_wave * (64 / b.group);
+    int w = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+    if (b.wcap > 0) w = std::min(w, b.wcap);
+    return w;
 }
 
 /* Launch routing and the bin chain over the columns whose patterns are in
@@ -590,6 +655,16 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     int* mq = ctx->d_queues + (size_t)(kMaxBins + 1) * n;
     HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 64 * sizeof(int), ctx->stream));
     a.status_or = 0;
+    /* HBM tie-path buffers of the LDS bins: one region per resident group */
+    size_t tie_need = 0;
+    for (const Bin& b : bins)
+        tie_need = std::max(tie_need, (size_t)ctx->num_cu * waves_per_cu(b) * (64 / b.group) * b.L.tie_per_group);
+    if (tie_need > ctx->tie_bytes) {
+        dfree(ctx->d_tie);
+        HIPCHK(hipMalloc((void**)&ctx->d_tie, tie_need));
+        ctx->tie_bytes = tie_need;
+    }
+    a.tie_scratch = ctx->d_tie;
     if (mixed) {
         launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->stream);
         HIPCHK(hipGetLastError());
@@ -602,8 +677,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             const int grp = bins[b].group;
             a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
             a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
-            size_t lds = L.per_wave * (64 / grp);
-            int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+            int per_cu = waves_per_cu(bins[b]);
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
                           ctx->num_cu * per_cu, ctx->stream);
             HIPCHK(hipGetLastError());
@@ -621,8 +695,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         const int grp = bins[b].group;
         a.overflow = fq + (size_t)(b + 1) * n;
         a.n_overflow = counts + b + 1;
-        size_t lds = L.per_wave * (64 / grp);
-        int per_cu = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+        int per_cu = waves_per_cu(bins[b]);
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());
@@ -726,9 +799,15 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     if (cap <= 0) return fail(ctx, MSPAI_ERR_ARG, "build: empty start pattern and no augmentation");
     if ((rc = ensure_outputs(ctx, cap))) return rc;
     const bool adaptive = ns > 0;
-    const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive);
+    const bool qrup = opts->qr == 2 && ns > 0 && opts->maxnew > 0;  // opt_level 3, src/imspai.cpp:483-491
+    const int needs = (qrup ? kNeedQrUpdate : 0) | (opts->precision == 1 && !qrup ? kNeedMixed : 0);
+    const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive, needs);
     std::vector<Bin> bins;
-    if (opts->binning) bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G);
+    if (opts->binning) {
+        int t = (opts->binning == 2 && ctx->tuned_table >= 0) ? ctx->tuned_table : 0;
+        bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t], needs);
+        ctx->last_table = t;
+    }
     int gwaves = 0;
     if ((rc = prepare_generic(ctx, G, &gwaves))) return rc;
 
@@ -749,7 +828,6 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     a.nsolve = ctx->d_nsolve;
     a.status = ctx->d_status;
     a.work = ctx->d_work;
-    const bool qrup = opts->qr == 2 && ns > 0 && opts->maxnew > 0;  // opt_level 3, src/imspai.cpp:483-491
     const bool mixed = opts->precision == 1 && !bins.empty() && !qrup;
     if (opts->cache > 0 && !qrup && opts->precision == 0) {
         const int cI = std::min(G.capI, 32), cJ = std::min(cap, 16);
@@ -844,9 +922,13 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
     int rc;
     const int n = ctx->ncols;
     const int cap = ctx->cap;
-    const ScratchLayout G = generic_layout(ctx, cap, 1, false);
+    const int needs = kNeedBackward | (opts->precision == 1 ? kNeedMixed : 0);
+    const ScratchLayout G = generic_layout(ctx, cap, 1, false, needs);
     std::vector<Bin> bins;
-    if (opts->binning) bins = bin_layouts(ctx, cap, 1, false, G);
+    if (opts->binning) {
+        int t = (opts->binning == 2 && ctx->tuned_table >= 0) ? ctx->tuned_table : 0;
+        bins = bin_layouts(ctx, cap, 1, false, G, kTables[t], needs);
+    }
     int gwaves = 0;
     if ((rc = prepare_generic(ctx, G, &gwaves))) return rc;
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -876,6 +958,42 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
         stats->nnz = ctx->nnz;
         stats->ncols = n;
         if ((rc = read_chain_counts(ctx, (int)bins.size(), mixed, stats))) return rc;
+    }
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeats, mspai_tune_report* report)
+{
+    if (!ctx || !opts) return MSPAI_ERR_ARG;
+    if (repeats < 1) repeats = 1;
+    mspai_opts o = *opts;
+    o.binning = 2;
+    int best = 0;
+    double best_ms = 1e300;
+    if (report) std::memset(report, 0, sizeof(*report));
+    for (int t = 0; t < kNumTables && t < MSPAI_MAX_TUNE; ++t) {
+        ctx->tuned_table = t;
+        double tmin = 1e300;
+        for (int r = 0; r < repeats + 1; ++r) {  // first run warms up
+            mspai_stats st;
+            int rc = mspai_build(ctx, &o, &st);
+            if (rc) {
+                ctx->tuned_table = -1;
+                return rc;
+            }
+            if (r > 0) tmin = std::min(tmin, st.solve_ms);
+        }
+        if (report) report->candidate_ms[t] = tmin;
+        if (tmin < best_ms) {
+            best_ms = tmin;
+            best = t;
+        }
+    }
+    ctx->tuned_table = best;
+    if (report) {
+        report->ncandidates = std::min(kNumTables, MSPAI_MAX_TUNE);
+        report->chosen = best;
+        std::snprintf(report->chosen_name, sizeof(report->chosen_name), "%s", kTables[best].name);
     }
     return MSPAI_OK;
 }

@@ -46,6 +46,8 @@ struct ScratchLayout {
     int capU;      // distinct candidates |U|
     int capSort;   // raw index lists before sorting (a power of two)
     int capPairs;  // candidates on the std::sort tie path
+    int tie_in_main;        // pairs/stack in the main region (else in BuildArgs::tie_scratch)
+    size_t tie_per_group;   // bytes of a group's tie region when not in the main one
 };
 
 /* Routing capacities of the size bins (|J| and raw row-list length). */
@@ -95,6 +97,7 @@ struct BuildArgs {
     int* status;
     double* work;
     char* scratch;          // per-wave HBM working sets of the generic kernel
+    char* tie_scratch;      // per-group HBM tie-path buffers of the LDS bins
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }

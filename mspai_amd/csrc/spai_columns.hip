@@ -59,7 +59,7 @@ struct WaveScratch {
     double* yq;    // Q^T b work vector in storage row order
 };
 
-__device__ __forceinline__ WaveScratch wave_scratch(char* ws, const ScratchLayout& L)
+__device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const ScratchLayout& L)
 {
     WaveScratch s;
     s.J = (int*)(ws + L.J);
@@ -72,9 +72,9 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, const ScratchLayou
     s.tau = (double*)(ws + L.tau);
     s.jt = (int*)(ws + L.jt);
     s.rho = (double*)(ws + L.rho);
-    s.pairs = (RhoPair*)(ws + L.pairs);
+    s.pairs = (RhoPair*)(tie + L.pairs);
     s.sel = (int*)(ws + L.sel);
-    s.stack = (int*)(ws + L.stack);
+    s.stack = (int*)(tie + L.stack);
     s.aq32 = (float*)(ws + L.aqr);
     s.bq32 = (float*)(ws + L.bq32);
     s.g = (double*)(ws + L.g);
@@ -1331,7 +1331,8 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
     const int g = Group<G>::index();
     char* base = kLds ? smem + (size_t)g * L.per_wave
                       : a.scratch + ((size_t)blockIdx.x * NG + g) * L.per_wave;
-    WaveScratch s = wave_scratch(base, L);
+    char* tie = L.tie_in_main ? base : a.tie_scratch + ((size_t)blockIdx.x * NG + g) * L.tie_per_group;
+    WaveScratch s = wave_scratch(base, tie, L);
     const int nq = *nqueue;
     while (true) {
         int q0 = 0;

@@ -204,3 +204,20 @@ def test_cache_is_bitwise_neutral(oracle, gpu, make):
     print(f"[cache {prob.name}] hits {stats['n_cache_hits']} of {int(info['nsolve'].sum())} solves", flush=True)
     if prob.name in ("laplace2d_static", "convdiff2d_adaptive", "laplace3d_probing"):
         assert stats["n_cache_hits"] > 0.5 * info["nsolve"].sum()
+
+
+def test_autotuned_bins_bitwise(oracle, gpu):
+    """The autotuner's choice of size bins changes speed only."""
+    prob = gen.config3(n=16384, bw=800)
+    ref = oracle.spai(prob, threads=THREADS)
+    b = gpu.Builder(0)
+    b.upload(prob.A, prob.row_pattern())
+    b.set_pattern(prob.P)
+    opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+    rep = b.autotune(repeats=1, **opts)
+    print(f"[autotune] chose {rep['name']} from {rep['candidate_ms']}", flush=True)
+    b.build(binning=2, **opts)
+    M = b.download()
+    info = b.column_info()
+    b.close()
+    assert_bitwise(oracle, prob, ref, M, info)

@@ -6,7 +6,9 @@ upwind convection-diffusion operator, 256 x 256 grid (n = 65,536), eps 0.4,
 step = one complete build of M (every column of M) on a device-resident
 copy of A; inputs are uploaded before the timed region.
 
-Prints one JSON line.  `value` is the columns/s of the FP64 build.
+Prints one JSON line.  `value` is the columns/s of the mixed-precision build
+(FP32 QR + FP64 refinement); the FP64 build is timed beside it.
+`--workload` selects another of the five workloads for profiling runs.
 `roofline` refers to the column kernel (HIP events on the library's stream);
 `cpu_baseline` times the CPU FP64 restatement of the reference on this host.
 """
@@ -24,6 +26,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# workload name -> (generator in mspai_amd.gen, description)
+WORKLOADS = {
+    "laplace2d": ("config1", "static SPAI, 2D 5-point Laplacian {nx}x{nx} (n={n}), P=pattern(A)"),
+    "convdiff2d": ("config2", "adaptive SPAI, 2D upwind convection-diffusion {nx}x{nx} (n={n}), eps=0.4, ns=5, "
+                              "mn=5, J0=diag"),
+    "random": ("config3", "adaptive SPAI, random nonsymmetric band matrix (n={n}), eps=0.3, ns=8, mn=4, J0=diag"),
+    "probing3d": ("config4", "MSPAI probing, 3D 7-point Laplacian {nx}^3 (n={n}), rho=100, P=pattern(A)"),
+    "varcoef3d": ("config5", "static SPAI, 3D variable-coefficient FV {nx}^3 (n={n}), P=pattern(A^2)"),
+}
 
 
 def _algorithmic_bytes(work, nnzM, n):
@@ -43,10 +55,19 @@ def _traffic(name):
         return None
 
 
-def _fro_residual(A, M):
-    R = (A @ M).tocsc()
-    R = R - __import__("scipy.sparse", fromlist=["identity"]).identity(A.shape[1], format="csc")
+def _fro_residual(A, M, B=None):
+    """||A M - B||_F with B = I when no target is given."""
+    import scipy.sparse as sps
+    R = (A @ M).tocsc() - (B if B is not None else sps.identity(A.shape[1], format="csc"))
     return float(np.sqrt((R.multiply(R)).sum()))
+
+
+def _column_rel_err(M, Mref):
+    """max over columns of ||m_k - m_k^ref|| / ||m_k^ref|| (same pattern)."""
+    d = (M - Mref).tocsc()
+    num = np.sqrt(np.asarray(d.multiply(d).sum(axis=0)).ravel())
+    den = np.sqrt(np.asarray(Mref.multiply(Mref).sum(axis=0)).ravel())
+    return float(np.max(num / np.maximum(den, 1e-300)))
 
 
 def _bwd_fwd(mspai_amd, gen, torch):
@@ -81,7 +102,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=None, help="grid size (default: the workload's full size)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="convdiff2d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="timed paths; with both, value is the mixed path")
@@ -112,11 +134,21 @@ def main():
     import mspai_amd
     from mspai_amd import gen
 
-    prob = gen.config2(args.nx)
+    make = getattr(gen, WORKLOADS[args.workload][0])
+    if args.workload == "random":
+        prob = make(n=args.nx) if args.nx else make()
+    else:
+        prob = make(args.nx) if args.nx else make()
     n = prob.n
+    if args.workload == "random":
+        nx = n
+    else:
+        nx = args.nx or (round(n ** 0.5) if args.workload in ("laplace2d", "convdiff2d") else round(n ** (1 / 3)))
     b = mspai_amd.Builder(0)
     b.upload(prob.A, prob.row_pattern())
     b.set_pattern(prob.P)
+    if prob.B is not None:
+        b.set_target(prob.B)
     base_opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
                      use_mean=prob.use_mean)
     # one-time set-up outside the timed region: choose the size bins
@@ -141,7 +173,7 @@ def main():
     rmx = timed(1) if args.precision in ("mixed", "both") else None
     # the reference's LRU cache option (-cs): same bits, repeated A(I,J) blocks reuse factors
     cache_run = None
-    if args.precision == "both":
+    if args.precision == "both" and args.workload == "convdiff2d":
         opts_c = dict(base_opts, precision=0, binning=2, cache=65536)
         for _ in range(args.warmup):
             b.build(**opts_c)
@@ -152,7 +184,8 @@ def main():
                      "cache_hits_per_build": stc["n_cache_hits"],
                      "pattern_equal_fp64": bool(np.array_equal(b.download().indices, r64["M"].indices))}
     b.close()
-    bwd = _bwd_fwd(mspai_amd, gen, torch) if (torch is not None and args.precision == "both") else None
+    bwd = (_bwd_fwd(mspai_amd, gen, torch)
+           if (torch is not None and args.precision == "both" and args.workload == "convdiff2d") else None)
     main_run = rmx if rmx is not None else r64
     M = main_run["M"]
     ms_per_step = main_run["ms_per_step"]
@@ -160,15 +193,19 @@ def main():
     kern_ms = main_run["kern_ms"]
     alg_bytes = _algorithmic_bytes(work, M.nnz, n)
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    traffic = _traffic("convdiff2d_256")
-    fro_gpu = _fro_residual(prob.A, M)
+    traffic = _traffic(f"{args.workload}_{nx}")
+    fro_gpu = _fro_residual(prob.A, M, prob.B)
     M64 = r64["M"]
     mixed_check = None
     if rmx is not None:
         same_pat = bool(np.array_equal(M64.indptr, M.indptr) and np.array_equal(M64.indices, M.indices))
         rel = float(np.max(np.abs(M.data - M64.data) / np.maximum(np.abs(M64.data), 1e-300))) if same_pat else None
-        mixed_check = {"pattern_equal_fp64": same_pat, "max_rel_err_vs_fp64": rel,
-                       "fro_AM_minus_I_fp64": _fro_residual(prob.A, M64),
+        fro64 = _fro_residual(prob.A, M64, prob.B)
+        mixed_check = {"pattern_equal_fp64": same_pat,
+                       "max_column_rel_err_vs_fp64": _column_rel_err(M, M64) if same_pat else None,
+                       "max_entry_rel_err_vs_fp64": rel,
+                       "fro_AM_minus_I_fp64": fro64,
+                       "fro_rel_diff": abs(fro_gpu - fro64) / fro64,
                        "n_fp64_fallback": rmx["n_fallback"]}
 
     cpu = None
@@ -194,7 +231,7 @@ def main():
                   "fp64_values_bitwise": bool(np.array_equal(Mref.data.view(np.int64), M64.data.view(np.int64))),
                   "reported_pattern_equal": bool(np.array_equal(Mref.indptr, M.indptr)
                                                  and np.array_equal(Mref.indices, M.indices)),
-                  "fro_AM_minus_I_oracle": _fro_residual(prob.A, Mref)}
+                  "fro_AM_minus_I_oracle": _fro_residual(prob.A, Mref, prob.B)}
         cpu = {"value": n / cpu_t, "unit": "columns/s", "cores": cores, "kind": "port",
                "sample": f"full workload (n={n}) x {reps} runs on {cores} threads; "
                          f"1 thread: {n / cpu1:.0f} columns/s",
@@ -214,8 +251,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 QR + f64 refinement" if rmx is not None else "f64",
         "data": "synthetic",
-        "config": {"workload": f"adaptive SPAI, 2D upwind convection-diffusion {args.nx}x{args.nx} "
-                               f"(n={n}), eps=0.4, ns=5, mn=5, J0=diag",
+        "config": {"workload": WORKLOADS[args.workload][1].format(nx=nx, n=n),
                    "n_columns": n, "nnz_A": int(prob.A.nnz), "nnz_M": int(M.nnz),
                    "precision": "mixed" if rmx is not None else "fp64",
                    "parallelism": "single GPU, persistent column queue"},

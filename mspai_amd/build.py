@@ -17,6 +17,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmspai_gpu.so")
+CLI = os.path.join(HERE, "mspai_gpu")
+CLI_SOURCES = ["host/mspai_cli.cpp", "host/mmio.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["capi.hip", "spai_columns.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -44,9 +46,22 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, OUT)
+    build_cli()
     if verbose:
-        print("built", OUT)
+        print("built", OUT, "and", CLI)
     return OUT
+
+
+def build_cli() -> str:
+    """The mspai_gpu command-line builder (host C++ over the C ABI)."""
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", *[os.path.join(HERE, s) for s in CLI_SOURCES],
+           "-L", HERE, "-lmspai_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

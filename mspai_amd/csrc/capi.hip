@@ -116,6 +116,7 @@ struct mspai_ctx {
     int* d_queues = nullptr;   // (kMaxBins + 1) queues of ncols entries
     int* d_counter = nullptr;  // FP64 queues: [0..8) lengths, [8..16) heads; [16..24) routed lengths;
                                // mixed queues: [24..32) lengths, [32..40) heads
+    int* h_counter = nullptr;  // pinned host copy of d_counter, [63] = nnz(M) of the last build
     double* d_wsum = nullptr;  // work-counter totals
     char* d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -203,6 +204,7 @@ extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
             return MSPAI_ERR_HIP;
         }
     if (hipMalloc((void**)&ctx->d_counter, 64 * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_counter, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&ctx->d_wsum, kNumWork * sizeof(double)) != hipSuccess) {
         delete ctx;
         return MSPAI_ERR_HIP;
@@ -239,6 +241,7 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_Mval);
     dfree(ctx->d_queues);
     dfree(ctx->d_counter);
+    if (ctx->h_counter) (void)hipHostFree(ctx->h_counter);
     dfree(ctx->d_wsum);
     dfree(ctx->d_scratch);
     dfree(ctx->d_tie);
@@ -504,6 +507,10 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.jt = take(capU * 4);
     L.rho = take(capU * 8);
     L.sel = take((size_t)std::max(1, maxnew) * 4);
+    L.segs = take(((size_t)capJ + 1) * 4);
+    L.segoff = take(((size_t)capJ + 1) * 4);
+    L.sid = take(capSort * 4);
+    L.wsum = take(8 * 8);
     if (needs & (kNeedMixed | kNeedBackward)) {
         L.bq32 = take(capB * 4);
         L.g = take((size_t)capJ * 8);
@@ -559,10 +566,10 @@ struct Bin {
 };
 
 /* A size-bin table: per bin the lane-group width, the |J|, |I|, distinct
- * candidate, raw-list and tie-path capacities, and a cap on resident waves
- * per CU (0 = as many as LDS allows, at most 32). */
+ * candidate and raw-list capacities, and a cap on resident waves per CU
+ * (0 = as many as LDS allows, at most 32). */
 struct BinEdge {
-    int group, J, I, U, sort, pairs, wcap;
+    int group, J, I, U, sort, wcap;
 };
 struct BinTable {
     const char* name;
@@ -573,17 +580,17 @@ struct BinTable {
 /* Candidate tables of the autotuner; table 0 is the default. */
 static const BinTable kTables[] = {
     {"g16-g32-g64-g64", 4,
-     {{16, 8, 16, 64, 128, 64, 0}, {32, 16, 32, 128, 256, 64, 0}, {64, 32, 64, 256, 512, 128, 0},
-      {64, 40, 128, 512, 1024, 128, 0}}},
+     {{16, 8, 16, 64, 128, 0}, {32, 16, 32, 128, 256, 0}, {64, 32, 64, 256, 512, 0},
+      {64, 40, 128, 512, 1024, 0}}},
     {"g32-g64-g64", 3,
-     {{32, 16, 32, 128, 256, 64, 0}, {64, 32, 64, 256, 512, 128, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
+     {{32, 16, 32, 128, 256, 0}, {64, 32, 64, 256, 512, 0}, {64, 40, 128, 512, 1024, 0}}},
     {"g16wide-g64-g64", 3,
-     {{16, 16, 16, 128, 256, 64, 0}, {64, 32, 64, 256, 512, 128, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
-    {"g64-g64", 2, {{64, 32, 64, 256, 512, 128, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
-    {"g16-g64", 2, {{16, 8, 16, 64, 128, 64, 0}, {64, 40, 128, 512, 1024, 128, 0}}},
+     {{16, 16, 16, 128, 256, 0}, {64, 32, 64, 256, 512, 0}, {64, 40, 128, 512, 1024, 0}}},
+    {"g64-g64", 2, {{64, 32, 64, 256, 512, 0}, {64, 40, 128, 512, 1024, 0}}},
+    {"g16-g64", 2, {{16, 8, 16, 64, 128, 0}, {64, 40, 128, 512, 1024, 0}}},
     {"g16-g32-g64-g64/occ12", 4,
-     {{16, 8, 16, 64, 128, 64, 12}, {32, 16, 32, 128, 256, 64, 12}, {64, 32, 64, 256, 512, 128, 8},
-      {64, 40, 128, 512, 1024, 128, 4}}},
+     {{16, 8, 16, 64, 128, 12}, {32, 16, 32, 128, 256, 12}, {64, 32, 64, 256, 512, 8},
+      {64, 40, 128, 512, 1024, 4}}},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 
@@ -601,13 +608,11 @@ static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, b
         size_t capI = std::min<size_t>(G.capI, e.I);
         size_t capU = adaptive ? std::min<size_t>(G.capU, e.U) : 1;
         size_t capSort = std::min<size_t>(G.capSort, e.sort);
-        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, std::min<size_t>(capU, e.pairs), needs,
-                                      false);
+        /* ties of scores across the cut are ordered in the generic kernel only */
+        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, needs, false);
         if (L.per_wave * (64 / e.group) > 64 * 1024) continue;
         out.push_back({e.group, e.wcap, L});
-        if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU) &&
-            L.capPairs >= G.capPairs)
-            break;
+        if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU)) break;
     }
     return out;
 }
@@ -621,12 +626,16 @@ static int ensure_scratch(mspai_ctx* ctx, size_t need)
     return MSPAI_OK;
 }
 
-/* Resident waves per CU for a bin: what 160 KB of LDS allows, at most 32,
- * and at most the bin's cap. */
-static int waves_per_cu(const Bin& b)
+/* Resident waves per CU for a bin's kernel in `mode`: what its registers
+ * and 160 KB of LDS allow (runtime occupancy query), at most 32, and at most
+ * the bin's cap.  A persistent launch of more waves than fit only adds
+ * waves that find the queue drained. */
+static int waves_per_cu(const Bin& b, int mode)
 {
     size_t lds = b.L.per_wave * (64 / b.group);
     int w = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
+    const int occ = binned_blocks_per_cu(b.L, b.group, mode);
+    if (occ > 0) w = std::min(w, occ);
     if (b.wcap > 0) w = std::min(w, b.wcap);
     return w;
 }
@@ -658,7 +667,10 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     /* HBM tie-path buffers of the LDS bins: one region per resident group */
     size_t tie_need = 0;
     for (const Bin& b : bins)
-        tie_need = std::max(tie_need, (size_t)ctx->num_cu * waves_per_cu(b) * (64 / b.group) * b.L.tie_per_group);
+        tie_need = std::max(tie_need, (size_t)ctx->num_cu *
+                                          std::max(waves_per_cu(b, fp64_mode),
+                                                   mixed ? waves_per_cu(b, fp64_mode == 2 ? 3 : 1) : 0) *
+                                          (64 / b.group) * b.L.tie_per_group);
     if (tie_need > ctx->tie_bytes) {
         dfree(ctx->d_tie);
         HIPCHK(hipMalloc((void**)&ctx->d_tie, tie_need));
@@ -677,7 +689,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             const int grp = bins[b].group;
             a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
             a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
-            int per_cu = waves_per_cu(bins[b]);
+            int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
                           ctx->num_cu * per_cu, ctx->stream);
             HIPCHK(hipGetLastError());
@@ -695,7 +707,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         const int grp = bins[b].group;
         a.overflow = fq + (size_t)(b + 1) * n;
         a.n_overflow = counts + b + 1;
-        int per_cu = waves_per_cu(bins[b]);
+        int per_cu = waves_per_cu(bins[b], fp64_mode);
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());
@@ -707,11 +719,19 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     return MSPAI_OK;
 }
 
-/* Read the queue counters of the last chain into stats and the context. */
+/* Queue the copy of the counters to the pinned host array (after the last
+ * kernel of a build on the stream; read once the stream is synchronized). */
+static int fetch_counters(mspai_ctx* ctx)
+{
+    HIPCHK(hipMemcpyAsync(ctx->h_counter, ctx->d_counter, 64 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    return MSPAI_OK;
+}
+
+/* Read the queue counters of the last chain (fetched with fetch_counters)
+ * into stats and the context. */
 static int read_chain_counts(mspai_ctx* ctx, int nbins, bool mixed, mspai_stats* stats)
 {
-    int cnt[64];
-    HIPCHK(hipMemcpy(cnt, ctx->d_counter, sizeof(cnt), hipMemcpyDeviceToHost));
+    const int* cnt = ctx->h_counter;
     ctx->last_nbins = nbins;
     stats->n_overflow = 0;
     stats->n_fallback = 0;
@@ -856,11 +876,12 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     hipLaunchKernelGGL(scatter_columns_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, cap,
                        ctx->d_m_len, ctx->d_m_idx, ctx->d_m_val, ctx->d_Mptr, ctx->d_Mind, ctx->d_Mval);
     HIPCHK(hipGetLastError());
+    launch_copy_ints(ctx->d_Mptr + n, ctx->d_counter + 63, 1, ctx->stream);
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    if ((rc = fetch_counters(ctx))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
 
-    int last = 0;
-    HIPCHK(hipMemcpy(&last, ctx->d_Mptr + n, sizeof(int), hipMemcpyDeviceToHost));
+    const int last = ctx->h_counter[63];
     ctx->nnzM = last;
     ctx->last_mixed = mixed;
     ctx->last_binning = opts->binning;
@@ -874,9 +895,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         stats->nnz = last;
         stats->ncols = n;
         if ((rc = read_chain_counts(ctx, (int)bins.size(), mixed, stats))) return rc;
-        int hits = 0;
-        HIPCHK(hipMemcpy(&hits, ctx->d_counter + 48, sizeof(int), hipMemcpyDeviceToHost));
-        stats->n_cache_hits = a.cache_slots > 0 ? hits : 0;
+        stats->n_cache_hits = a.cache_slots > 0 ? ctx->h_counter[48] : 0;
     }
     return MSPAI_OK;
 }
@@ -947,6 +966,7 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
     if ((rc = run_chain(ctx, a, bins, G, gwaves, mixed, 2, n))) return rc;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    if ((rc = fetch_counters(ctx))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));

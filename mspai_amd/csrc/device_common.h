@@ -40,6 +40,7 @@ struct DevCsc {
 struct ScratchLayout {
     size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y, z;
     size_t srow, spos, iold, sposold, jold, colid, reflrows, bperm, yq;
+    size_t segs, segoff, sid, wsum;
     size_t per_wave;
     int capJ;      // |J|
     int capI;      // |I| (also the leading dimension of A(I,J))
@@ -115,11 +116,45 @@ struct Group {
     static_assert(G == 16 || G == 32 || G == 64, "group width");
     static __device__ __forceinline__ int lane() { return threadIdx.x & (G - 1); }
     static __device__ __forceinline__ int index() { return (threadIdx.x & (kWave - 1)) / G; }
+    /* value of lane (l xor S) of the same group.  DPP within a row of 16:
+     * quad_perm [1,0,3,2] (S = 1), [2,3,0,1] (S = 2), row_half_mirror then
+     * quad_perm [3,2,1,0] (S = 4: 7 - (l xor 3) = l xor 4 within 8 lanes),
+     * row_ror:8 (S = 8); ds_swizzle xor mode within 32 lanes (S = 16);
+     * ds_bpermute across the halves (S = 32). */
+    template <int S>
+    static __device__ __forceinline__ int xor_lane(int v)
+    {
+        if (S == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+        if (S == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+        if (S == 4) return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false), 0x1B, 0xF,
+                                                    0xF, false);
+        if (S == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);
+        if (S == 16) return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+        return __shfl_xor(v, S, kWave);
+    }
+    template <int S>
+    static __device__ __forceinline__ double xor_lane(double v)
+    {
+        long long b = __double_as_longlong(v);
+        int lo = xor_lane<S>((int)(b & 0xffffffffll));
+        int hi = xor_lane<S>((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+    template <int S>
+    static __device__ __forceinline__ double sum_step(double p)
+    {
+        if (S >= G) return p;
+        return p + xor_lane<S>(p);
+    }
     /* the fixed-order sum: p_l <- p_l + p_(l xor s), s = G/2 .. 1 */
     static __device__ __forceinline__ double sum_fixed(double p)
     {
-#pragma unroll
-        for (int s = G / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s, G);
+        p = sum_step<32>(p);
+        p = sum_step<16>(p);
+        p = sum_step<8>(p);
+        p = sum_step<4>(p);
+        p = sum_step<2>(p);
+        p = sum_step<1>(p);
         return p;
     }
     static __device__ __forceinline__ float sum_f(float p)

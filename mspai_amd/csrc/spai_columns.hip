@@ -21,6 +21,10 @@
  * Columns come from a persistent work queue (one atomic per column), the
  * device form of Load_Balance::Grab_M_Col (src/Load_Balance.imp:34-69).
  */
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "device_common.h"
 #include "spai_host.h"
 
@@ -57,6 +61,10 @@ struct WaveScratch {
     int* reflrows; // rows a reflector spans (rows present when it was made)
     double* bperm; // right-hand side in storage row order
     double* yq;    // Q^T b work vector in storage row order
+    int* segs;     // colptr[J[t]] of the current J
+    int* segoff;   // first flat entry of column J[t]; segoff[|J|] = total
+    int* sid;      // flat entry -> position t in J
+    double* wsum;  // the column's work counters
 };
 
 __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const ScratchLayout& L)
@@ -89,6 +97,10 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const S
     s.reflrows = (int*)(ws + L.reflrows);
     s.bperm = (double*)(ws + L.bperm);
     s.yq = (double*)(ws + L.yq);
+    s.segs = (int*)(ws + L.segs);
+    s.segoff = (int*)(ws + L.segoff);
+    s.sid = (int*)(ws + L.sid);
+    s.wsum = (double*)(ws + L.wsum);
     return s;
 }
 
@@ -108,28 +120,61 @@ constexpr double kGuard = 1e-10;
 constexpr float kCondGuard = 1e-4f;
 constexpr int kRefineSteps = 3;
 
+/* Column segments of the current J: every lane t < |J| reads colptr[J[t]]
+ * and colptr[J[t] + 1] at once, an exclusive scan gives each column's first
+ * flat entry, and sid maps a flat entry back to its column.  Gathers over
+ * the flat entries then issue all their loads together instead of one
+ * column after another.  Returns the number of entries; sid is written only
+ * when it fits (<= cap). */
+template <int G>
+__device__ __forceinline__ int load_segments(const BuildArgs& a, const WaveScratch& s, int n, int cap)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    int carry = 0;
+    for (int base = 0; base < n; base += G) {
+        const int t = base + lane;
+        int start = 0, cnt = 0;
+        if (t < n) {
+            const int j = s.J[t];
+            start = a.A.colptr[j];
+            cnt = a.A.colptr[j + 1] - start;
+        }
+        const int incl = Gr::scan_incl(cnt);
+        if (t < n) {
+            s.segs[t] = start;
+            s.segoff[t] = carry + incl - cnt;
+        }
+        carry += __shfl(incl, G - 1, G);
+    }
+    if (lane == 0) s.segoff[n] = carry;
+    __syncthreads();
+    if (carry <= cap) {
+        for (int t = lane; t < n; t += G) {
+            const int o = s.segoff[t], e = s.segoff[t + 1];
+            for (int k = o; k < e; ++k) s.sid[k] = t;
+        }
+    }
+    __syncthreads();
+    return carry;
+}
+
 /* Get_I_Set (src/Spai_Sub.imp:65-122): sorted distinct rows of the columns in
- * J.  Returns |I|, or kOverflow when the raw list or I exceeds the layout. */
+ * J.  Returns |I|, or kOverflow when the raw list or I exceeds the layout.
+ * Leaves the column segments of J for gather_ahat. */
 template <int G>
 __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int n)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
-    int off = 0;
-    for (int jj = 0; jj < n; ++jj) {
-        int j = s.J[jj];
-        off += a.A.colptr[j + 1] - a.A.colptr[j];
-    }
-    if (off > L.capSort) return kOverflow;
-    off = 0;
-    for (int jj = 0; jj < n; ++jj) {
-        int j = s.J[jj];
-        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-        for (int e = lane; e < cnt; e += G) s.sbuf[off + e] = a.A.rowind[e0 + e];
-        off += cnt;
+    const int total = load_segments<G>(a, s, n, L.capSort);
+    if (total > L.capSort) return kOverflow;
+    for (int e = lane; e < total; e += G) {
+        const int t = s.sid[e];
+        s.sbuf[e] = a.A.rowind[s.segs[t] + e - s.segoff[t]];
     }
     __syncthreads();
-    int m = group_sort_unique<G>(s.sbuf, off, s.sbuf);
+    int m = group_sort_unique<G>(s.sbuf, total, s.sbuf);
     if (m > L.capI) return kOverflow;
     for (int i = lane; i < m; i += G) s.I[i] = s.sbuf[i];
     __syncthreads();
@@ -137,7 +182,8 @@ __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& 
 }
 
 /* Create_Submatrix_AHat (src/Spai_Sub.imp:221-282): A(I,J) column-major with
- * leading dimension capI, straight into the QR working array. */
+ * leading dimension capI, straight into the QR working array.  Uses the
+ * column segments build_I left for this J. */
 template <int G, bool kMixed>
 __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int m, int n)
@@ -153,16 +199,15 @@ __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLay
                 s.aqr[jj * ld + i] = 0.0;
         }
     __syncthreads();
-    for (int jj = 0; jj < n; ++jj) {
-        int j = s.J[jj];
-        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-        for (int e = lane; e < cnt; e += G) {
-            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-            if (kMixed)
-                s.aq32[jj * ld + pos] = (float)a.A.val[e0 + e];
-            else
-                s.aqr[jj * ld + pos] = a.A.val[e0 + e];
-        }
+    const int total = s.segoff[n];
+    for (int e = lane; e < total; e += G) {
+        const int t = s.sid[e];
+        const int src = s.segs[t] + e - s.segoff[t];
+        const int pos = lower_bound_i(s.I, m, a.A.rowind[src]);
+        if (kMixed)
+            s.aq32[t * ld + pos] = (float)a.A.val[src];
+        else
+            s.aqr[t * ld + pos] = a.A.val[src];
     }
     __syncthreads();
 }
@@ -439,28 +484,125 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
     return kFallback;  // not converged in kRefineSteps: let FP64 decide
 }
 
-/* Residual_Norm (src/Spai_Sub.imp:915-941): r = A(I,J) x - b in dgemv's column
- * order with beta = -1, formed from the sparse columns (a stored entry of
- * column j updates its row in order of j, as the dense product does; the
- * dense product's zero terms leave every nonzero residual unchanged), then
- * sqrt of the sequential sum of squares. */
-template <int G>
-__device__ __forceinline__ double residual_norm(const BuildArgs& a, const WaveScratch& s, int m, int n)
+/* Element j of a register row (j uniform in the group, any value). */
+template <int NMAX>
+__device__ __forceinline__ double row_pick(const double (&a)[NMAX], int j)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < NMAX; ++t)
+        if (t == j) v = a[t];
+    return v;
+}
+
+/* Approximative_Solution (src/Spai_Unrestrained.imp:135-195) with row r of
+ * A(I,J) held in the registers of group lane r (|I| <= G, |J| <= NMAX).  The
+ * operations and their order are those of ls_solve (a single row per lane
+ * is the 64-lane order's only slot); the matrix never goes through LDS and
+ * the group needs no barrier between reflectors. */
+template <int G, int NMAX>
+__device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
-    for (int i = lane; i < m; i += G) s.r[i] = -s.b0[i];
+    const int ld = L.capI;
+    const bool row = lane < m;
+    double a[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
+    double b = row ? s.b[lane] : 0.0;
+    const int k = m < n ? m : n;
+    for (int i = 0; i < k; ++i) {
+        const double ai = row_pick(a, i);
+        const double alpha = __shfl(ai, i, G);
+        const double xn2 = Gr::sum_fixed(lane > i ? __fma_rn(ai, ai, 0.0) : 0.0);
+        if (i + 1 < m && xn2 != 0.0) {
+            const double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
+            const double beta = -copysign(nrm, alpha);
+            const double tau = (beta - alpha) / beta;
+            const double scal = 1.0 / (alpha - beta);
+            const double vi = lane > i ? ai * scal : (lane == i ? 1.0 : 0.0);
+            const double ni = lane > i ? vi : (lane == i ? beta : ai);
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j)
+                if (j == i) a[j] = ni;
+            const double mt = -tau;
+#pragma unroll
+            for (int j = 1; j < NMAX; ++j) {
+                if (j > i && j < n) {
+                    const double w = Gr::sum_fixed(lane >= i ? __fma_rn(vi, a[j], 0.0) : 0.0);
+                    if (lane >= i) a[j] = __fma_rn(vi, mt * w, a[j]);
+                }
+            }
+            const double wb = Gr::sum_fixed(lane >= i ? __fma_rn(vi, b, 0.0) : 0.0);
+            if (lane >= i) b = __fma_rn(vi, mt * wb, b);
+        }
+    }
+    int st = 0;
+    if (m < n) {
+        st = COL_UNDERDET;
+        b = 0.0;
+    } else {
+        const double d = row_pick(a, lane);
+        if (Gr::ballot(lane < n && d == 0.0) != 0ull) {
+            st = COL_SINGULAR;
+        } else {
+            for (int K = n - 1; K >= 0; --K) {
+                double xk = __shfl(b, K, G);
+                const double rk = row_pick(a, K);
+                const double rkk = __shfl(rk, K, G);
+                if (xk != 0.0) {
+                    xk = xk / rkk;
+                    if (lane == K) b = xk;
+                    if (lane < K) b = __fma_rn(-xk, rk, b);
+                }
+            }
+        }
+    }
+    if (lane < n) s.b[lane] = b;
     __syncthreads();
-    for (int jj = 0; jj < n; ++jj) {
-        double xj = s.b[jj];
-        if (xj == 0.0) continue;
-        int j = s.J[jj];
-        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-        for (int e = lane; e < cnt; e += G) {
-            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-            s.r[pos] = __fma_rn(xj, a.A.val[e0 + e], s.r[pos]);
+    return st;
+}
+
+/* Residual_Norm (src/Spai_Sub.imp:915-941): r = A(I,J) x - b in dgemv's column
+ * order with beta = -1, then sqrt of the sequential sum of squares.  With
+ * `dense`, from A(I,J) in the working set; else from the sparse columns (a
+ * stored entry of column j updates its row in order of j, as the dense
+ * product does; the dense product's zero terms leave every nonzero residual
+ * unchanged). */
+template <int G>
+__device__ __forceinline__ double residual_norm(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
+                                                int m, int n, bool dense)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    if (dense) {
+        /* A(I,J) is still intact in LDS (the register QR leaves it): dgemv's
+         * own column order, r_i = fma(x_j, A_ij, r_i) for each x_j != 0 */
+        const int ld = L.capI;
+        for (int i = lane; i < m; i += G) {
+            double r = -s.b0[i];
+            for (int jj = 0; jj < n; ++jj) {
+                const double xj = s.b[jj];
+                if (xj != 0.0) r = __fma_rn(xj, s.aqr[jj * ld + i], r);
+            }
+            s.r[i] = r;
         }
         __syncthreads();
+    } else {
+        for (int i = lane; i < m; i += G) s.r[i] = -s.b0[i];
+        __syncthreads();
+        for (int jj = 0; jj < n; ++jj) {
+            double xj = s.b[jj];
+            if (xj == 0.0) continue;
+            int j = s.J[jj];
+            int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+            for (int e = lane; e < cnt; e += G) {
+                int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+                s.r[pos] = __fma_rn(xj, a.A.val[e0 + e], s.r[pos]);
+            }
+            __syncthreads();
+        }
     }
     double sum = 0.0;
     if (lane == 0) {
@@ -474,8 +616,12 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const WaveSc
 }
 
 /* Augment_Sparsity (src/Spai_Sub.imp:1010-1116) with Union_Nl_Sets and
- * Compute_Rhos.  Returns the new |J|, -1 when J~ is empty, or kOverflow. */
-template <int G, bool kMixed>
+ * Compute_Rhos.  Returns the new |J|, -1 when J~ is empty, or kOverflow.
+ * Without kTies a tie of scores across the cut also returns kOverflow: the
+ * std::sort emulation that orders such ties lives only in the generic
+ * kernel, which keeps the size bins' register budget small (ties at the cut
+ * are rare: none in the five workloads, a handful in the CSparse tests). */
+template <int G, bool kMixed, bool kTies>
 __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int col,
                                        int m, int n, double rn, int* st, double* wk)
 {
@@ -624,7 +770,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         }
     } else {
         /* a tie straddles the cut: reproduce std::sort's order */
-        if (c > L.capPairs) return kOverflow;
+        if (!kTies || c > L.capPairs) return kOverflow;
         if (lane == 0) {
             for (int t = 0; t < c; ++t) {
                 s.pairs[t].idx = s.jt[t];
@@ -660,17 +806,14 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
 template <int G>
 __device__ __forceinline__ void count_solve(const BuildArgs& a, const WaveScratch& s, int m, int n, double* wk)
 {
+    (void)a;
     if (!wk || Group<G>::lane() != 0) return;
     double dm = (double)m, dn = (double)n;
     double kk = dm < dn ? dm : dn;
     wk[0] = __dadd_rn(wk[0], __dsub_rn(2.0 * dm * dn * dn, (2.0 / 3.0) * kk * kk * kk));
     wk[1] = __dadd_rn(wk[1], (4.0 * dm * dn - 2.0 * dn * dn) + dn * dn + 2.0 * dm * dn);
-    double g = 0.0;
-    for (int jj = 0; jj < n; ++jj) {
-        int j = s.J[jj];
-        g += 8.0 + 12.0 * (a.A.colptr[j + 1] - a.A.colptr[j]);
-    }
-    wk[2] += g;
+    /* sum over J of (8 + 12 nnz(a_j)): exact integers */
+    wk[2] += 8.0 * dn + 12.0 * (double)s.segoff[n];
     wk[6] += dm;
     wk[7] += dn;
 }
@@ -876,14 +1019,14 @@ __device__ __forceinline__ double make_reflector(double* ai, int c, int rows)
  * right-hand side and back-substitutes row by row (sequential, unfused, as
  * qrupdate_voll does); the solution is put back in ascending column order.
  */
-template <int G>
+template <int G, bool kTies>
 __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                                  int col)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
     const int ld = L.capI;
-    double* wk = a.work ? a.work + (size_t)col * kNumWork : nullptr;
+    double* wk = a.work ? s.wsum : nullptr;  // summed in LDS, stored with the column
     int p0 = a.p_ptr[col], n = a.p_ptr[col + 1] - p0;
     if (n > L.capJ) return kMoveOn;
     if (wk && lane < kNumWork) wk[lane] = 0.0;
@@ -950,7 +1093,7 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
     }
     ++solves;
     count_solve<G>(a, s, m, n, wk);
-    double rn = residual_norm<G>(a, s, m, n);
+    double rn = residual_norm<G>(a, L, s, m, n, false);
     int rows = m, cols = n;
     for (int step = 0; step < a.ns && rn > a.epsilon; ++step) {
         /* keep J, I and the storage positions of I */
@@ -961,7 +1104,7 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
         }
         __syncthreads();
         const int nold = n, mold = m;
-        int nn = augment<G, false>(a, L, s, col, m, n, rn, &st, wk);
+        int nn = augment<G, false, kTies>(a, L, s, col, m, n, rn, &st, wk);
         if (nn == kOverflow) return kMoveOn;
         if (nn < 0) break;
         n = nn;
@@ -1070,7 +1213,7 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
         __syncthreads();
         ++solves;
         count_solve<G>(a, s, m, n, wk);
-        rn = residual_norm<G>(a, s, m, n);
+        rn = residual_norm<G>(a, L, s, m, n, false);
         if (step == a.ns - 1) break;
     }
     size_t o = (size_t)col * a.cap;
@@ -1085,6 +1228,7 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
         a.status[col] = st | a.status_or;
     }
     __syncthreads();
+    if (a.work && lane < kNumWork) a.work[(size_t)col * kNumWork + lane] = s.wsum[lane];
     return kDone;
 }
 
@@ -1254,13 +1398,13 @@ __device__ __forceinline__ int ls_solve_cached(const BuildArgs& a, const Scratch
     return st_solve;
 }
 
-template <int G, bool kMixed>
+template <int G, bool kMixed, int NMAX, bool kTies>
 __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int col)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
-    double* wk = a.work ? a.work + (size_t)col * kNumWork : nullptr;
+    double* wk = a.work ? s.wsum : nullptr;  // summed in LDS, stored with the column
     int p0 = a.p_ptr[col], n = a.p_ptr[col + 1] - p0;
     if (n > L.capJ) return kMoveOn;
     if (wk && lane < kNumWork) wk[lane] = 0.0;
@@ -1273,19 +1417,23 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         if (m == kOverflow) return kMoveOn;
         gather_ahat<G, kMixed>(a, L, s, m, n);
         st |= make_rhs<G>(a, s, m, col);
+        bool dense = false;
         if (kMixed) {
             if (ls_solve_mixed<G>(a, L, s, m, n) == kFallback) return kRedoFp64;
         } else if (a.cache_slots > 0) {
             st |= ls_solve_cached<G>(a, L, s, m, n);
+        } else if (NMAX > 0 && m <= G && n <= NMAX) {
+            st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 1)>(L, s, m, n);
+            dense = true;
         } else {
             st |= ls_solve<G>(L, s, m, n);
         }
         ++solves;
         count_solve<G>(a, s, m, n, wk);
-        rn = residual_norm<G>(a, s, m, n);
+        rn = residual_norm<G>(a, L, s, m, n, dense);
         if (kMixed && !(fabs(rn - a.epsilon) > kGuard * a.epsilon)) return kRedoFp64;
         if (rn <= a.epsilon || step == a.ns - 1) break;
-        int nn = augment<G, kMixed>(a, L, s, col, m, n, rn, &st, wk);
+        int nn = augment<G, kMixed, kTies>(a, L, s, col, m, n, rn, &st, wk);
         if (nn == kOverflow) return kMoveOn;
         if (nn == kFallback) return kRedoFp64;
         if (nn < 0) break;
@@ -1303,6 +1451,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         a.status[col] = st | a.status_or;
     }
     __syncthreads();
+    if (a.work && lane < kNumWork) a.work[(size_t)col * kNumWork + lane] = s.wsum[lane];
     return kDone;
 }
 
@@ -1320,7 +1469,7 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
 
 /* kMode: 0 FP64 build, 1 mixed build, 2 FP64 adjoint, 3 mixed adjoint,
  * 4 FP64 build with the dense QR update (-qr 2). */
-template <int G, bool kLds, int kMode>
+template <int G, bool kLds, int kMode, int NMAX>
 __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
 {
@@ -1335,8 +1484,13 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
     WaveScratch s = wave_scratch(base, tie, L);
     const int nq = *nqueue;
     while (true) {
+        /* a plain load first: once the queue is drained (or empty) the waves
+         * leave without queueing on the atomic */
         int q0 = 0;
-        if (lane_id() == 0) q0 = atomicAdd(head, NG * kRounds);
+        if (lane_id() == 0) {
+            q0 = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (q0 < nq) q0 = atomicAdd(head, NG * kRounds);
+        }
         q0 = __shfl(q0, 0, kWave);
         if (q0 >= nq) break;
         for (int round = 0; round < kRounds; ++round) {
@@ -1346,11 +1500,11 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
             if (q < nq) {
                 col = queue[q];
                 if (kMode < 2)
-                    outcome = solve_column<G, kMode == 1>(a, L, s, col);
+                    outcome = solve_column<G, kMode == 1, NMAX, !kLds>(a, L, s, col);
                 else if (kMode < 4)
                     outcome = backward_column<G, kMode == 3>(a, L, s, col);
                 else
-                    outcome = solve_column_qrup<G>(a, L, s, col);
+                    outcome = solve_column_qrup<G, !kLds>(a, L, s, col);
             }
             __syncthreads();
             const bool leader = Group<G>::lane() == 0;
@@ -1394,31 +1548,53 @@ __global__ void copy_ints_kernel(const int* src, int* dst, int count)
     if (i < count) dst[i] = src[i];
 }
 
+using ColumnKernel = void (*)(BuildArgs, ScratchLayout, const int*, const int*, int*);
+
+/* The LDS column kernel for a bin: register-resident QR (FP64 build) when a
+ * group holds every row of the bin (|I| <= G) and the bin's |J| fits the
+ * unrolled width. */
+static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
+{
+    const bool regs = mode == 0 && L.capI <= group && L.capJ <= (group == 16 ? 8 : (group == 32 ? 16 : 32));
+#define MSPAI_PICK(MODE)                                                                            \
+    if (group == 16) return &spai_column_kernel<16, true, MODE, 0>;                                 \
+    if (group == 32) return &spai_column_kernel<32, true, MODE, 0>;                                 \
+    return &spai_column_kernel<64, true, MODE, 0>
+    switch (mode) {
+    case 0:
+        if (group == 16) return regs ? &spai_column_kernel<16, true, 0, 8> : &spai_column_kernel<16, true, 0, 0>;
+        if (group == 32) return regs ? &spai_column_kernel<32, true, 0, 16> : &spai_column_kernel<32, true, 0, 0>;
+        return regs ? &spai_column_kernel<64, true, 0, 32> : &spai_column_kernel<64, true, 0, 0>;
+    case 1: MSPAI_PICK(1);
+    case 2: MSPAI_PICK(2);
+    case 3: MSPAI_PICK(3);
+    default: MSPAI_PICK(4);
+    }
+#undef MSPAI_PICK
+}
+
+int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, size_t>, int> memo;
+    const ColumnKernel k = binned_kernel(L, group, mode);
+    const size_t lds = L.per_wave * (kWave / group);
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple((const void*)k, lds);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k, kWave, lds) != hipSuccess) nb = 0;
+    memo[key] = nb;
+    return nb;
+}
+
 void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mode, const int* queue,
                    const int* nqueue, int* head, int nwaves, hipStream_t stream)
 {
-    size_t lds = L.per_wave * (kWave / group);
-#define MSPAI_LAUNCH(G, MODE)                                                                                    \
-    hipLaunchKernelGGL((spai_column_kernel<G, true, MODE>), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue, \
-                       nqueue, head)
-#define MSPAI_LAUNCH_G(MODE)      \
-    do {                          \
-        if (group == 16)          \
-            MSPAI_LAUNCH(16, MODE); \
-        else if (group == 32)     \
-            MSPAI_LAUNCH(32, MODE); \
-        else                      \
-            MSPAI_LAUNCH(64, MODE); \
-    } while (0)
-    switch (mode) {
-    case 0: MSPAI_LAUNCH_G(0); break;
-    case 1: MSPAI_LAUNCH_G(1); break;
-    case 2: MSPAI_LAUNCH_G(2); break;
-    case 3: MSPAI_LAUNCH_G(3); break;
-    default: MSPAI_LAUNCH_G(4); break;
-    }
-#undef MSPAI_LAUNCH_G
-#undef MSPAI_LAUNCH
+    const size_t lds = L.per_wave * (kWave / group);
+    hipLaunchKernelGGL(binned_kernel(L, group, mode), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue, nqueue,
+                       head);
 }
 
 void launch_classify(const BuildArgs& a, int nbins, const BinEdges& edges, int* queues, int* counts,
@@ -1437,13 +1613,13 @@ void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, int mode, c
                          const int* nqueue, int* head, int nwaves, hipStream_t stream)
 {
     if (mode == 2)
-        hipLaunchKernelGGL((spai_column_kernel<64, false, 2>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 2, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
                            nqueue, head);
     else if (mode == 4)
-        hipLaunchKernelGGL((spai_column_kernel<64, false, 4>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 4, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
                            nqueue, head);
     else
-        hipLaunchKernelGGL((spai_column_kernel<64, false, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 0, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
                            nqueue, head);
 }
 

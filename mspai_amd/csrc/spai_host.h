@@ -19,6 +19,10 @@ namespace mspai {
 void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mode, const int* queue,
                    const int* nqueue, int* head, int nwaves, hipStream_t stream);
 
+/* Workgroups (one wave each) of that kernel that fit on one CU at once, by
+ * its registers and LDS (0 if the runtime cannot tell). */
+int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode);
+
 /* Route columns to bin queues: queue b < nbins at queues + b*n with its
  * length in counts[b]; the rest to gqueue (length gcount). */
 void launch_classify(const BuildArgs& a, int nbins, const BinEdges& edges, int* queues, int* counts,

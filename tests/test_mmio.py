@@ -45,3 +45,20 @@ def test_negative_values_get_one_space():
     M = sp.csc_matrix(np.array([[-1.5, 0.0], [2.0, 3.25e-7]]))
     text = format_mtx(M)
     assert "1 1 -1.5000000000000e+00\n" in text and "2 1  2.0000000000000e+00\n" in text
+
+
+CLI = os.path.join(ROOT, "mspai_amd", "mspai_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="mspai_gpu is built by __graft_entry__.build()")
+def test_cli_option_errors_exit_1(tmp_path):
+    """Option and input errors are reported before any device call."""
+    r = subprocess.run([CLI], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    src = tmp_path / "a.mtx"
+    write_input_mtx(src, sp.identity(4, format="csc"))
+    for args, msg in ((["-xx", "1"], "unknown option -xx"), (["-pp", "3"], "-pp"), (["-ep"], "needs a value")):
+        r = subprocess.run([CLI, str(src), *args], capture_output=True, text=True)
+        assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
+    r = subprocess.run([CLI, str(tmp_path / "missing.mtx")], capture_output=True, text=True)
+    assert r.returncode == 1 and "mspai_gpu:" in r.stderr

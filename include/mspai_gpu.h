@@ -93,6 +93,13 @@ int mspai_update_values(mspai_ctx* ctx, const double* val);
  * the diagonal pattern (-pp 1, src/Pattern.cpp:378-423). */
 int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* rowind);
 
+/* Start pattern = the pattern of the last M built in this context: the
+ * P_Memory reuse of repeated setups (P_Memory = M_REAL->To_Pattern,
+ * src/imspai.cpp:850-854; between pattern recomputations the reference
+ * re-solves on it with maxnew = max_impr_steps = 0, src/imspai.cpp:663-677).
+ * Combine with mspai_update_values and a build with max_impr_steps = 0. */
+int mspai_reuse_pattern(mspai_ctx* ctx);
+
 /* Target B (nrows x ncols CSC) for MSPAI probing (Create_bk_Hat,
  * src/Spai_Sub.imp:735-775); colptr == NULL restores the identity. */
 int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* rowind, const double* val);
@@ -168,6 +175,17 @@ int mspai_work_totals(mspai_ctx* ctx, double* totals);
  * overflow from bin b-1); bin nbins is the generic kernel.  Arrays hold
  * nbins + 1 entries (at most 8). */
 int mspai_bin_counts(const mspai_ctx* ctx, int* nbins, int* routed, int* queued);
+
+/* Sparse product on device vectors: y = op(X) x with X = A (which = 0, the
+ * uploaded nrows x ncols matrix) or X = M (which = 1, the last built
+ * preconditioner), op = transpose when `transpose` is nonzero.  This is the
+ * preconditioner application of the Krylov solve (PC_MSPAI::Apply_MSPAI =
+ * MatMult(PM, x, y), src/imspai.cpp:47-53) and the operator product beside
+ * it.  Every y_i sums its terms in increasing index order with separate
+ * multiply and add.  `stream` (a hipStream_t, or NULL) orders the product
+ * after the caller's work on that stream; with NULL the call runs on the
+ * context's stream and returns when y is written. */
+int mspai_apply(mspai_ctx* ctx, int which, int transpose, const double* d_x, double* d_y, void* stream);
 
 #ifdef __cplusplus
 }

@@ -20,11 +20,12 @@ NWORK = 8
 COL_SINGULAR, COL_UNDERDET, COL_NAN_RHO, COL_FALLBACK, COL_K_NOT_IN_I = 1, 2, 4, 8, 16
 
 EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspai_last_error",
-            "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_set_target",
+            "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_reuse_pattern",
+            "mspai_set_target",
             "mspai_build", "mspai_result_nnz", "mspai_download_csc",
             "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts",
             "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
-            "mspai_backward", "mspai_autotune"]
+            "mspai_backward", "mspai_autotune", "mspai_apply"]
 
 
 class Opts(C.Structure):
@@ -132,6 +133,11 @@ class Builder:
         self._chk(self.lib.mspai_set_pattern(self.h, _p(_i32(P.indptr)), _p(_i32(P.indices))),
                   "mspai_set_pattern")
 
+    def reuse_pattern(self):
+        """Start pattern = the last built M's pattern (the reference's P_Memory
+        re-solve between pattern recomputations)."""
+        self._chk(self.lib.mspai_reuse_pattern(self.h), "mspai_reuse_pattern")
+
     def set_target(self, B=None):
         if B is None:
             self._chk(self.lib.mspai_set_target(self.h, None, None, None), "mspai_set_target")
@@ -156,6 +162,14 @@ class Builder:
 
     def update_target_device(self, ptr: int):
         self._chk(self.lib.mspai_update_target_device(self.h, C.c_void_p(ptr)), "mspai_update_target_device")
+
+    def apply_device(self, which: int, transpose: bool, x_ptr: int, y_ptr: int, stream_ptr: int = 0):
+        """y = op(A) x (which 0) or op(M) x (which 1) on device vectors; with a
+        stream handle the product is queued on that stream, else it runs on
+        the context's stream and returns when y is written."""
+        self._chk(self.lib.mspai_apply(self.h, int(which), int(bool(transpose)), C.c_void_p(x_ptr),
+                                       C.c_void_p(y_ptr), C.c_void_p(stream_ptr) if stream_ptr else None),
+                  "mspai_apply")
 
     def result_values_device(self, ptr: int):
         self._chk(self.lib.mspai_result_values_device(self.h, C.c_void_p(ptr)), "mspai_result_values_device")

@@ -110,7 +110,8 @@ struct mspai_ctx {
     double* d_work = nullptr;
     int *d_Mptr = nullptr, *d_Mind = nullptr;
     double* d_Mval = nullptr;
-    long long nnzM = -1;
+    long long nnzM = -1;        // nnz of the current M (-1: none, or its values are stale)
+    long long pattern_nnz = -1;  // nnz of the last built M's pattern (kept across value updates)
     size_t Mcap = 0;
     // queues and scratch
     int* d_queues = nullptr;   // (kMaxBins + 1) queues of ncols entries
@@ -124,6 +125,11 @@ struct mspai_ctx {
     size_t tie_bytes = 0;
     void* d_cub = nullptr;
     size_t cub_bytes = 0;
+    // CSR copies for mspai_apply (built on first use; valid while *_csr_ok)
+    int *d_acsr_ptr = nullptr, *d_acsr_ind = nullptr, *d_mcsr_ptr = nullptr, *d_mcsr_ind = nullptr;
+    double *d_acsr_val = nullptr, *d_mcsr_val = nullptr;
+    int* d_csr_cursor = nullptr;
+    bool a_csr_ok = false, m_csr_ok = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     /* pattern->R cache */
     int cache_slots = 0, cache_capI = 0, cache_capJ = 0;
@@ -252,6 +258,13 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_cache_status);
     dfree(ctx->d_cache_data);
     if (ctx->d_cub) (void)hipFree(ctx->d_cub);
+    dfree(ctx->d_acsr_ptr);
+    dfree(ctx->d_acsr_ind);
+    dfree(ctx->d_acsr_val);
+    dfree(ctx->d_mcsr_ptr);
+    dfree(ctx->d_mcsr_ind);
+    dfree(ctx->d_mcsr_val);
+    dfree(ctx->d_csr_cursor);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -314,6 +327,8 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     ctx->maxcolnnz = maxc;
     ctx->maxrownnz = maxr;
     ctx->nnzM = -1;
+    ctx->a_csr_ok = ctx->m_csr_ok = false;
+    ctx->pattern_nnz = -1;
     ctx->tuned_table = -1;
     if (!ctx->pattern_set) {
         dfree(ctx->d_p_ptr);
@@ -329,6 +344,7 @@ extern "C" int mspai_update_values(mspai_ctx* ctx, const double* val)
     HIPCHK(hipMemcpyAsync(ctx->d_val, val, ctx->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
+    ctx->a_csr_ok = ctx->m_csr_ok = false;
     return MSPAI_OK;
 }
 
@@ -365,6 +381,18 @@ extern "C" int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* r
     ctx->plen_max = pmax;
     ctx->pattern_set = true;
     return MSPAI_OK;
+}
+
+extern "C" int mspai_reuse_pattern(mspai_ctx* ctx)
+{
+    if (!ctx) return MSPAI_ERR_ARG;
+    if (ctx->pattern_nnz < 0) return fail(ctx, MSPAI_ERR_STATE, "reuse_pattern: build M first");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<int> ptr(ctx->ncols + 1), ind(std::max<long long>(ctx->pattern_nnz, 1));
+    HIPCHK(hipMemcpy(ptr.data(), ctx->d_Mptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (ctx->pattern_nnz > 0)
+        HIPCHK(hipMemcpy(ind.data(), ctx->d_Mind, ctx->pattern_nnz * sizeof(int), hipMemcpyDeviceToHost));
+    return mspai_set_pattern(ctx, ptr.data(), ind.data());
 }
 
 extern "C" int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* rowind, const double* val)
@@ -883,6 +911,8 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
 
     const int last = ctx->h_counter[63];
     ctx->nnzM = last;
+    ctx->pattern_nnz = last;
+    ctx->m_csr_ok = false;
     ctx->last_mixed = mixed;
     ctx->last_binning = opts->binning;
     if (stats) {
@@ -907,6 +937,7 @@ extern "C" int mspai_update_values_device(mspai_ctx* ctx, const double* d_val)
     HIPCHK(hipMemcpyAsync(ctx->d_val, d_val, ctx->nnz * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
+    ctx->a_csr_ok = ctx->m_csr_ok = false;
     return MSPAI_OK;
 }
 
@@ -1074,3 +1105,69 @@ extern "C" int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int*
     if (work) HIPCHK(hipMemcpy(work, ctx->d_work, n * kNumWork * sizeof(double), hipMemcpyDeviceToHost));
     return MSPAI_OK;
 }
+
+/* CSR copy of a CSC matrix (nrows x ncols) on the stream: row counts, scan,
+ * fill, rows sorted by column. */
+static int build_csr(mspai_ctx* ctx, int nrows, int ncols, long long nnz, const int* colptr, const int* rowind,
+                     const double* val, int*& rp, int*& ri, double*& rv, hipStream_t st)
+{
+    dfree(rp);
+    dfree(ri);
+    dfree(rv);
+    dfree(ctx->d_csr_cursor);
+    HIPCHK(hipMalloc((void**)&rp, (nrows + 1) * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&ri, std::max<long long>(nnz, 1) * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&rv, std::max<long long>(nnz, 1) * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&ctx->d_csr_cursor, std::max(nrows, 1) * sizeof(int)));
+    size_t need = 0;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, need, ctx->d_csr_cursor, rp + 1, nrows, st));
+    if (need > ctx->cub_bytes) {
+        if (ctx->d_cub) (void)hipFree(ctx->d_cub);
+        ctx->d_cub = nullptr;
+        HIPCHK(hipMalloc(&ctx->d_cub, need));
+        ctx->cub_bytes = need;
+    }
+    launch_fill_int(ctx->d_csr_cursor, nrows, 0, st);
+    launch_fill_int(rp, 1, 0, st);
+    launch_row_counts(rowind, nnz, ctx->d_csr_cursor, st);
+    size_t bytes = ctx->cub_bytes;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(ctx->d_cub, bytes, ctx->d_csr_cursor, rp + 1, nrows, st));
+    launch_fill_int(ctx->d_csr_cursor, nrows, 0, st);
+    launch_csr_fill(colptr, rowind, val, ncols, rp, ctx->d_csr_cursor, ri, rv, st);
+    launch_csr_sort_rows(nrows, rp, ri, rv, st);
+    HIPCHK(hipGetLastError());
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_apply(mspai_ctx* ctx, int which, int transpose, const double* d_x, double* d_y, void* stream)
+{
+    if (!ctx || !d_x || !d_y || (which != 0 && which != 1)) return MSPAI_ERR_ARG;
+    if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "apply: upload the matrix first");
+    if (which == 1 && ctx->nnzM < 0) return fail(ctx, MSPAI_ERR_STATE, "apply: build M first");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const int n = ctx->ncols;
+    const int nrows = which == 0 ? ctx->nrows : n;
+    const int* cp = which == 0 ? ctx->d_colptr : ctx->d_Mptr;
+    const int* ri = which == 0 ? ctx->d_rowind : ctx->d_Mind;
+    const double* cv = which == 0 ? ctx->d_val : ctx->d_Mval;
+    int rc;
+    if (transpose) {
+        launch_line_dot(n, cp, ri, cv, d_x, d_y, st);
+    } else {
+        bool& ok = which == 0 ? ctx->a_csr_ok : ctx->m_csr_ok;
+        int*& rp = which == 0 ? ctx->d_acsr_ptr : ctx->d_mcsr_ptr;
+        int*& rind = which == 0 ? ctx->d_acsr_ind : ctx->d_mcsr_ind;
+        double*& rv = which == 0 ? ctx->d_acsr_val : ctx->d_mcsr_val;
+        if (!ok) {
+            const long long nnz = which == 0 ? ctx->nnz : ctx->nnzM;
+            if ((rc = build_csr(ctx, nrows, n, nnz, cp, ri, cv, rp, rind, rv, st))) return rc;
+            ok = true;
+        }
+        launch_line_dot(nrows, rp, rind, rv, d_x, d_y, st);
+    }
+    HIPCHK(hipGetLastError());
+    if (!stream) HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSPAI_OK;
+}
+

@@ -140,10 +140,35 @@ struct Group {
         int hi = xor_lane<S>((int)(b >> 32));
         return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     }
+    /* p_l + p_(l xor S) for S = 32 and 16 with the gfx950 lane swaps:
+     * v_permlane32_swap of two copies of p leaves p_l in one and p_(l xor 32)
+     * in the other (lanes below 32), or the same pair the other way round
+     * (lanes above); v_permlane16_swap does the same for the 16-lane rows.
+     * Both are VALU moves, not LDS-pipe permutes. */
+    template <int S>
+    static __device__ __forceinline__ double swap_add(double p)
+    {
+        const long long bits = __double_as_longlong(p);
+        const unsigned lo = (unsigned)bits, hi = (unsigned)(bits >> 32);
+        unsigned x0, x1, y0, y1;
+        if (S == 32) {
+            auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+            auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+            x0 = l[0], x1 = h[0], y0 = l[1], y1 = h[1];
+        } else {
+            auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+            auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+            x0 = l[0], x1 = h[0], y0 = l[1], y1 = h[1];
+        }
+        const double x = __longlong_as_double((long long)(((unsigned long long)x1 << 32) | x0));
+        const double y = __longlong_as_double((long long)(((unsigned long long)y1 << 32) | y0));
+        return x + y;
+    }
     template <int S>
     static __device__ __forceinline__ double sum_step(double p)
     {
         if (S >= G) return p;
+        if (S >= 16) return swap_add<S>(p);
         return p + xor_lane<S>(p);
     }
     /* the fixed-order sum: p_l <- p_l + p_(l xor s), s = G/2 .. 1 */
@@ -156,6 +181,35 @@ struct Group {
         p = sum_step<2>(p);
         p = sum_step<1>(p);
         return p;
+    }
+    /* K independent fixed-order sums, step by step so that their moves and
+     * adds interleave */
+    template <int S, int K>
+    static __device__ __forceinline__ void sum_step_n(double (&p)[K])
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k) p[k] = sum_step<S>(p[k]);
+    }
+    template <int K>
+    static __device__ __forceinline__ void sum_fixed_n(double (&p)[K])
+    {
+        sum_step_n<32>(p);
+        sum_step_n<16>(p);
+        sum_step_n<8>(p);
+        sum_step_n<4>(p);
+        sum_step_n<2>(p);
+        sum_step_n<1>(p);
+    }
+    /* value of lane `src` of the group (src uniform in the group) */
+    static __device__ __forceinline__ double lane_value(double v, int src)
+    {
+        if (G == kWave) {
+            const long long bits = __double_as_longlong(v);
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, src);
+            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(bits >> 32), src);
+            return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        }
+        return __shfl(v, src, G);
     }
     static __device__ __forceinline__ float sum_f(float p)
     {

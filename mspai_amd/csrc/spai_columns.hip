@@ -499,22 +499,29 @@ __device__ __forceinline__ double row_pick(const double (&a)[NMAX], int j)
  * A(I,J) held in the registers of group lane r (|I| <= G, |J| <= NMAX).  The
  * operations and their order are those of ls_solve (a single row per lane
  * is the 64-lane order's only slot); the matrix never goes through LDS and
- * the group needs no barrier between reflectors. */
+ * the group needs no barrier between reflectors.  Slot NMAX of the row
+ * holds the right-hand side; a reflector is applied to four slots at a time
+ * so that their reductions interleave (slots outside (i, n) and b keep their
+ * values). */
 template <int G, int NMAX>
 __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
     using Gr = Group<G>;
+    constexpr int kB = 4;  // slots per block
     const int lane = Gr::lane();
     const int ld = L.capI;
     const bool row = lane < m;
-    double a[NMAX];
+    double a[NMAX + 1];
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
-    double b = row ? s.b[lane] : 0.0;
+    a[NMAX] = row ? s.b[lane] : 0.0;
     const int k = m < n ? m : n;
     for (int i = 0; i < k; ++i) {
-        const double ai = row_pick(a, i);
-        const double alpha = __shfl(ai, i, G);
+        double ai = 0.0;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t)
+            if (t == i) ai = a[t];
+        const double alpha = Gr::lane_value(ai, i);
         const double xn2 = Gr::sum_fixed(lane > i ? __fma_rn(ai, ai, 0.0) : 0.0);
         if (i + 1 < m && xn2 != 0.0) {
             const double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
@@ -524,33 +531,51 @@ __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveS
             const double vi = lane > i ? ai * scal : (lane == i ? 1.0 : 0.0);
             const double ni = lane > i ? vi : (lane == i ? beta : ai);
 #pragma unroll
-            for (int j = 0; j < NMAX; ++j)
-                if (j == i) a[j] = ni;
+            for (int t = 0; t < NMAX; ++t)
+                if (t == i) a[t] = ni;
             const double mt = -tau;
+            const bool act = lane >= i;
 #pragma unroll
-            for (int j = 1; j < NMAX; ++j) {
-                if (j > i && j < n) {
-                    const double w = Gr::sum_fixed(lane >= i ? __fma_rn(vi, a[j], 0.0) : 0.0);
-                    if (lane >= i) a[j] = __fma_rn(vi, mt * w, a[j]);
+            for (int jb = 0; jb <= NMAX; jb += kB) {
+                /* the block holds a slot in (i, n) or the right-hand side */
+                if ((jb + kB - 1 > i && jb < n) || jb + kB > NMAX) {
+                    double p[kB];
+#pragma unroll
+                    for (int t = 0; t < kB; ++t)
+                        p[t] = (jb + t <= NMAX && act) ? __fma_rn(vi, a[jb + t <= NMAX ? jb + t : NMAX], 0.0) : 0.0;
+                    Gr::sum_fixed_n(p);
+#pragma unroll
+                    for (int t = 0; t < kB; ++t) {
+                        const int j = jb + t;
+                        if (j <= NMAX) {
+                            const bool live = j == NMAX || (j > i && j < n);
+                            if (live && act) a[j] = __fma_rn(vi, mt * p[t], a[j]);
+                        }
+                    }
                 }
             }
-            const double wb = Gr::sum_fixed(lane >= i ? __fma_rn(vi, b, 0.0) : 0.0);
-            if (lane >= i) b = __fma_rn(vi, mt * wb, b);
         }
     }
+    double b = a[NMAX];
     int st = 0;
     if (m < n) {
         st = COL_UNDERDET;
         b = 0.0;
     } else {
-        const double d = row_pick(a, lane);
+        double d = 0.0;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t)
+            if (t == lane) d = a[t];
         if (Gr::ballot(lane < n && d == 0.0) != 0ull) {
             st = COL_SINGULAR;
         } else {
             for (int K = n - 1; K >= 0; --K) {
-                double xk = __shfl(b, K, G);
-                const double rk = row_pick(a, K);
-                const double rkk = __shfl(rk, K, G);
+                double rk = 0.0;
+#pragma unroll
+                for (int t = 0; t < NMAX; ++t)
+                    if (t == K) rk = a[t];
+                double xk = Gr::lane_value(b, K);
+                const double rkk = Gr::lane_value(rk, K);
                 if (xk != 0.0) {
                     xk = xk / rkk;
                     if (lane == K) b = xk;

@@ -36,6 +36,18 @@ void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream);
 void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, int mode, const int* queue,
                          const int* nqueue, int* head, int nwaves, hipStream_t stream);
 
+/* Sparse products (apply.hip). */
+void launch_fill_int(int* p, long long count, int value, hipStream_t stream);
+/* counts[r] += number of entries in row r (counts zeroed beforehand). */
+void launch_row_counts(const int* rowind, long long nnz, int* counts, hipStream_t stream);
+/* CSC -> CSR entries in rows given rowptr (cursor zeroed); rows unsorted. */
+void launch_csr_fill(const int* colptr, const int* rowind, const double* val, int ncols, const int* rowptr,
+                     int* cursor, int* colind, double* cval, hipStream_t stream);
+void launch_csr_sort_rows(int nrows, const int* rowptr, int* colind, double* cval, hipStream_t stream);
+/* y[i] = sum over line i of val * x[ind] in storage order (mul, then add). */
+void launch_line_dot(int nlines, const int* ptr, const int* ind, const double* val, const double* x, double* y,
+                     hipStream_t stream);
+
 }  // namespace mspai
 
 #endif

@@ -223,3 +223,29 @@ def test_autotuned_bins_bitwise(oracle, gpu):
     info = b.column_info()
     b.close()
     assert_bitwise(oracle, prob, ref, M, info)
+
+
+def test_reuse_pattern_static_resolve(oracle, gpu):
+    """Repeated setups: an adaptive build, new values on the same pattern of A,
+    then the static re-solve on the previous M's pattern (P_Memory) equals the
+    oracle's static build with that pattern, bit for bit."""
+    prob = gen.config2(64)
+    b = gpu.Builder(0)
+    b.upload(prob.A, prob.row_pattern())
+    b.set_pattern(prob.P)
+    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+    M1 = b.download()
+    rng = np.random.Generator(np.random.PCG64(11))
+    A2 = prob.A.copy()
+    A2.data = A2.data * (1.0 + 0.05 * rng.standard_normal(A2.nnz))
+    b.update_values(A2.data)
+    b.reuse_pattern()
+    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=0)
+    M2 = b.download()
+    info = b.column_info()
+    b.close()
+    prob2 = gen.Problem("reuse", A2, gen.pattern_of(M1), None, epsilon=prob.epsilon, maxnew=prob.maxnew,
+                        max_impr_steps=0)
+    ref = oracle.spai(prob2, threads=THREADS)
+    assert np.array_equal(M2.indptr, M1.indptr) and np.array_equal(M2.indices, M1.indices)
+    assert_bitwise(oracle, prob2, ref, M2, info)

@@ -40,7 +40,7 @@ struct DevCsc {
 struct ScratchLayout {
     size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y, z;
     size_t srow, spos, iold, sposold, jold, colid, reflrows, bperm, yq;
-    size_t segs, segoff, sid, wsum;
+    size_t segs, segoff, sid, wsum, aq32;
     size_t per_wave;
     int capJ;      // |J|
     int capI;      // |I| (also the leading dimension of A(I,J))
@@ -200,6 +200,87 @@ struct Group {
         sum_step_n<2>(p);
         sum_step_n<1>(p);
     }
+    /* one v_permlane{16,32}_swap on the two halves of a pair of doubles */
+    template <int S>
+    static __device__ __forceinline__ void swap_pair(double& a, double& b)
+    {
+        const long long ab = __double_as_longlong(a), bb = __double_as_longlong(b);
+        unsigned al = (unsigned)ab, ah = (unsigned)(ab >> 32), bl = (unsigned)bb, bh = (unsigned)(bb >> 32);
+        if (S == 32) {
+            auto l = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+            auto h = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+            al = l[0], bl = l[1], ah = h[0], bh = h[1];
+        } else {
+            auto l = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+            auto h = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+            al = l[0], bl = l[1], ah = h[0], bh = h[1];
+        }
+        a = __longlong_as_double((long long)(((unsigned long long)ah << 32) | al));
+        b = __longlong_as_double((long long)(((unsigned long long)bh << 32) | bl));
+    }
+    /* Four fixed-order sums over a 64-lane group, every lane ending with all
+     * four.  The xor-32 and xor-16 steps of the four butterflies share their
+     * lane swaps: swapping p0 with p2 leaves lanes 0-31 the pair (p0_l,
+     * p0_(l+32)) and lanes 32-63 the pair (p2_(l-32), p2_l), so one add gives
+     * the xor-32 step of p0 in the lower and of p2 in the upper half; the
+     * same for p1/p3, then a 16-lane swap of the two halves leaves row r of
+     * 16 lanes with the xor-16 step of p_r.  The last four steps run on that
+     * one value; two rounds of swaps give every lane all four results.  Each
+     * sum is formed from the same pairs as sum_fixed (additions commute). */
+    static __device__ __forceinline__ void sum4_bcast(double (&p)[4])
+    {
+        static_assert(G == 64, "sum4_bcast needs the whole wave");
+        double a = p[0], c = p[2], b = p[1], d = p[3];
+        swap_pair<32>(a, c);
+        swap_pair<32>(b, d);
+        double qa = a + c, qb = b + d;
+        swap_pair<16>(qa, qb);
+        double q = qa + qb;
+        q = sum_step<8>(q);
+        q = sum_step<4>(q);
+        q = sum_step<2>(q);
+        q = sum_step<1>(q);
+        double u = q, v = q;
+        swap_pair<16>(u, v);  // u: rows 0/2 results, v: rows 1/3
+        double u2 = u, v2 = v;
+        swap_pair<32>(u, u2);  // u: p0 everywhere, u2: p2 everywhere
+        swap_pair<32>(v, v2);  // v: p1, v2: p3
+        p[0] = u, p[1] = v, p[2] = u2, p[3] = v2;
+    }
+    /* Two fixed-order sums over a 32-lane group (xor-16 step by one shared
+     * 16-lane swap), every lane ending with both. */
+    static __device__ __forceinline__ void sum2_bcast(double& p0, double& p1)
+    {
+        static_assert(G == 32, "sum2_bcast is for 32-lane groups");
+        double a = p0, b = p1;
+        swap_pair<16>(a, b);
+        double q = a + b;
+        q = sum_step<8>(q);
+        q = sum_step<4>(q);
+        q = sum_step<2>(q);
+        q = sum_step<1>(q);
+        double u = q, v = q;
+        swap_pair<16>(u, v);
+        p0 = u, p1 = v;
+    }
+    /* K sums (K a multiple of 4), every lane ending with all of them */
+    template <int K>
+    static __device__ __forceinline__ void sum_bcast_n(double (&p)[K])
+    {
+        if constexpr (G == 64) {
+#pragma unroll
+            for (int k = 0; k < K; k += 4) {
+                double t[4] = {p[k], p[k + 1], p[k + 2], p[k + 3]};
+                sum4_bcast(t);
+                p[k] = t[0], p[k + 1] = t[1], p[k + 2] = t[2], p[k + 3] = t[3];
+            }
+        } else if constexpr (G == 32) {
+#pragma unroll
+            for (int k = 0; k < K; k += 2) sum2_bcast(p[k], p[k + 1]);
+        } else {
+            sum_fixed_n(p);
+        }
+    }
     /* value of lane `src` of the group (src uniform in the group) */
     static __device__ __forceinline__ double lane_value(double v, int src)
     {
@@ -211,10 +292,26 @@ struct Group {
         }
         return __shfl(v, src, G);
     }
+    template <int S>
+    static __device__ __forceinline__ float sum_step_f(float p)
+    {
+        if (S >= G) return p;
+        const int bits = __float_as_int(p);
+        if (S >= 16) {
+            auto r = S == 32 ? __builtin_amdgcn_permlane32_swap(bits, bits, false, false)
+                             : __builtin_amdgcn_permlane16_swap(bits, bits, false, false);
+            return __int_as_float((int)r[0]) + __int_as_float((int)r[1]);
+        }
+        return p + __int_as_float(xor_lane<S>(bits));
+    }
     static __device__ __forceinline__ float sum_f(float p)
     {
-#pragma unroll
-        for (int s = G / 2; s >= 1; s >>= 1) p = p + __shfl_xor(p, s, G);
+        p = sum_step_f<32>(p);
+        p = sum_step_f<16>(p);
+        p = sum_step_f<8>(p);
+        p = sum_step_f<4>(p);
+        p = sum_step_f<2>(p);
+        p = sum_step_f<1>(p);
         return p;
     }
     static __device__ __forceinline__ double min(double p)

@@ -46,7 +46,7 @@ struct WaveScratch {
     RhoPair* pairs;
     int* sel;
     int* stack;
-    float* aq32;  // mixed path: FP32 copy of A(I,J) (aliases aqr)
+    float* aq32;  // mixed path: FP32 copy of A(I,J) (aqr keeps the FP64 one)
     float* bq32;  // mixed path: FP32 right-hand side
     double* g;    // mixed path: A(I,J)^T r
     double* y;    // mixed path: triangular-solve work vector; backward: m
@@ -83,7 +83,7 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const S
     s.pairs = (RhoPair*)(tie + L.pairs);
     s.sel = (int*)(ws + L.sel);
     s.stack = (int*)(tie + L.stack);
-    s.aq32 = (float*)(ws + L.aqr);
+    s.aq32 = (float*)(ws + L.aq32);
     s.bq32 = (float*)(ws + L.bq32);
     s.g = (double*)(ws + L.g);
     s.y = (double*)(ws + L.y);
@@ -182,8 +182,9 @@ __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& 
 }
 
 /* Create_Submatrix_AHat (src/Spai_Sub.imp:221-282): A(I,J) column-major with
- * leading dimension capI, straight into the QR working array.  Uses the
- * column segments build_I left for this J. */
+ * leading dimension capI, straight into the QR working array (and an FP32
+ * copy for the mixed path).  Uses the column segments build_I left for this
+ * J. */
 template <int G, bool kMixed>
 __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int m, int n)
@@ -193,10 +194,8 @@ __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLay
     const int ld = L.capI;
     for (int jj = 0; jj < n; ++jj)
         for (int i = lane; i < m; i += G) {
-            if (kMixed)
-                s.aq32[jj * ld + i] = 0.f;
-            else
-                s.aqr[jj * ld + i] = 0.0;
+            s.aqr[jj * ld + i] = 0.0;
+            if (kMixed) s.aq32[jj * ld + i] = 0.f;
         }
     __syncthreads();
     const int total = s.segoff[n];
@@ -204,10 +203,9 @@ __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLay
         const int t = s.sid[e];
         const int src = s.segs[t] + e - s.segoff[t];
         const int pos = lower_bound_i(s.I, m, a.A.rowind[src]);
-        if (kMixed)
-            s.aq32[t * ld + pos] = (float)a.A.val[src];
-        else
-            s.aqr[t * ld + pos] = a.A.val[src];
+        const double v = a.A.val[src];
+        s.aqr[t * ld + pos] = v;
+        if (kMixed) s.aq32[t * ld + pos] = (float)v;
     }
     __syncthreads();
 }
@@ -325,6 +323,96 @@ __device__ __forceinline__ int ls_solve(const ScratchLayout& L, const WaveScratc
     return 0;
 }
 
+/* v <- (R^T R)^-1 v in FP64 for the upper-triangular R stored column-major
+ * in q (leading dimension ld; FP64 or FP32 entries promoted to FP64). */
+template <int G, typename RT>
+__device__ __forceinline__ void normal_solve(const RT* q, int ld, double* v, int n)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    for (int K = 0; K < n; ++K) {  // R^T y = v
+        double yk = v[K] / (double)q[K * ld + K];
+        __syncthreads();
+        if (lane == 0) v[K] = yk;
+        for (int i = K + 1 + lane; i < n; i += G) v[i] = __fma_rn(-yk, (double)q[i * ld + K], v[i]);
+        __syncthreads();
+    }
+    for (int K = n - 1; K >= 0; --K) {  // R d = y
+        double dk = v[K] / (double)q[K * ld + K];
+        __syncthreads();
+        if (lane == 0) v[K] = dk;
+        for (int i = lane; i < K; i += G) v[i] = __fma_rn(-dk, (double)q[K * ld + i], v[i]);
+        __syncthreads();
+    }
+}
+
+/* out[i] = sum_j C(I_i, J_j) w_j over I in FP64, from the CSC columns or
+ * (dense) from A(I,J) in the working set; with `base`, out = base - C w. */
+template <int G>
+__device__ __forceinline__ void csc_apply(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int m,
+                                          int n, const double* w, const double* base, double* out, bool dense)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const double sign = base ? -1.0 : 1.0;
+    if (dense) {
+        /* from A(I,J) in the working set (FP64, untouched by the FP32 QR) */
+        const int ld = L.capI;
+        for (int i = lane; i < m; i += G) {
+            double o = base ? base[i] : 0.0;
+            for (int jj = 0; jj < n; ++jj) o = __fma_rn(sign * w[jj], s.aqr[jj * ld + i], o);
+            out[i] = o;
+        }
+        __syncthreads();
+        return;
+    }
+    for (int i = lane; i < m; i += G) out[i] = base ? base[i] : 0.0;
+    __syncthreads();
+    for (int jj = 0; jj < n; ++jj) {
+        double wj = sign * w[jj];
+        int j = s.J[jj];
+        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+        for (int e = lane; e < cnt; e += G) {
+            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+            out[pos] = __fma_rn(wj, a.A.val[e0 + e], out[pos]);
+        }
+        __syncthreads();
+    }
+}
+
+/* out[jj] = (A(I,J)^T v)_jj, from the sparse columns or (dense) from A(I,J)
+ * in the working set. */
+template <int G>
+__device__ __forceinline__ void csc_apply_t(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int m,
+                                            int n, const double* v, double* out, bool dense)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    if (dense) {
+        const int ld = L.capI;
+        for (int jj = 0; jj < n; ++jj) {
+            double p = 0.0;
+            for (int i = lane; i < m; i += G) p = __fma_rn(s.aqr[jj * ld + i], v[i], p);
+            p = Gr::sum_fixed(p);
+            if (lane == 0) out[jj] = p;
+        }
+        __syncthreads();
+        return;
+    }
+    for (int jj = 0; jj < n; ++jj) {
+        int j = s.J[jj];
+        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
+        double p = 0.0;
+        for (int e = lane; e < cnt; e += G) {
+            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+            p = __fma_rn(a.A.val[e0 + e], v[pos], p);
+        }
+        p = Gr::sum_fixed(p);
+        if (lane == 0) out[jj] = p;
+    }
+    __syncthreads();
+}
+
 /* c <- (I - tau v v^T) c in FP32 (any summation order). */
 template <int G>
 __device__ __forceinline__ void apply_reflector_f(const float* v, float* c, int i, int m, float tau)
@@ -424,47 +512,9 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
     for (int i = lane; i < n; i += G) s.b[i] = (double)s.bq32[i];
     __syncthreads();
     for (int it = 0; it < kRefineSteps; ++it) {
-        /* r = b0 - A x over I (FP64) */
-        for (int i = lane; i < m; i += G) s.r[i] = s.b0[i];
-        __syncthreads();
-        for (int jj = 0; jj < n; ++jj) {
-            double xj = s.b[jj];
-            int j = s.J[jj];
-            int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-            for (int e = lane; e < cnt; e += G) {
-                int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-                s.r[pos] = __fma_rn(-xj, a.A.val[e0 + e], s.r[pos]);
-            }
-            __syncthreads();
-        }
-        /* g = A^T r */
-        for (int jj = 0; jj < n; ++jj) {
-            int j = s.J[jj];
-            int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-            double p = 0.0;
-            for (int e = lane; e < cnt; e += G) {
-                int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-                p = __fma_rn(a.A.val[e0 + e], s.r[pos], p);
-            }
-            p = Gr::sum_fixed(p);
-            if (lane == 0) s.g[jj] = p;
-        }
-        __syncthreads();
-        /* R^T y = g (forward), R d = y (backward); R from the FP32 factor */
-        for (int K = 0; K < n; ++K) {
-            double yk = s.g[K] / (double)q[K * ld + K];
-            __syncthreads();
-            if (lane == 0) s.g[K] = yk;
-            for (int i = K + 1 + lane; i < n; i += G) s.g[i] = __fma_rn(-yk, (double)q[i * ld + K], s.g[i]);
-            __syncthreads();
-        }
-        for (int K = n - 1; K >= 0; --K) {
-            double dk = s.g[K] / (double)q[K * ld + K];
-            __syncthreads();
-            if (lane == 0) s.g[K] = dk;
-            for (int i = lane; i < K; i += G) s.g[i] = __fma_rn(-dk, (double)q[K * ld + i], s.g[i]);
-            __syncthreads();
-        }
+        csc_apply<G>(a, L, s, m, n, s.b, s.b0, s.r, true);  // r = b0 - A x (FP64)
+        csc_apply_t<G>(a, L, s, m, n, s.r, s.g, true);      // g = A^T r
+        normal_solve<G>(q, ld, s.g, n);                      // R^T R d = g, R from the FP32 factor
         double dmx = 0.0, xmx = 0.0;
         for (int i = lane; i < n; i += G) {
             double x = s.b[i] + s.g[i];
@@ -508,6 +558,7 @@ __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveS
 {
     using Gr = Group<G>;
     constexpr int kB = 4;  // slots per block
+    static_assert(NMAX % kB == 0, "register width");
     const int lane = Gr::lane();
     const int ld = L.capI;
     const bool row = lane < m;
@@ -516,44 +567,58 @@ __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveS
     for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
     a[NMAX] = row ? s.b[lane] : 0.0;
     const int k = m < n ? m : n;
-    for (int i = 0; i < k; ++i) {
-        double ai = 0.0;
+    bool zero_diag = false;  // R_ii == 0 on this lane's row (final once step i is done)
+    /* step i = ib + r: the block of i is known at compile time, so column i is
+     * picked from four registers, and only the blocks from ib on are visited */
 #pragma unroll
-        for (int t = 0; t < NMAX; ++t)
-            if (t == i) ai = a[t];
-        const double alpha = Gr::lane_value(ai, i);
-        const double xn2 = Gr::sum_fixed(lane > i ? __fma_rn(ai, ai, 0.0) : 0.0);
-        if (i + 1 < m && xn2 != 0.0) {
-            const double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
-            const double beta = -copysign(nrm, alpha);
-            const double tau = (beta - alpha) / beta;
-            const double scal = 1.0 / (alpha - beta);
-            const double vi = lane > i ? ai * scal : (lane == i ? 1.0 : 0.0);
-            const double ni = lane > i ? vi : (lane == i ? beta : ai);
+    for (int ib = 0; ib < NMAX; ib += kB) {
+        if (ib >= k) break;
+#pragma unroll 1
+        for (int r = 0; r < kB; ++r) {
+            const int i = ib + r;
+            if (i >= k) break;
+            double ai = a[ib];
 #pragma unroll
-            for (int t = 0; t < NMAX; ++t)
-                if (t == i) a[t] = ni;
-            const double mt = -tau;
-            const bool act = lane >= i;
+            for (int t = 1; t < kB; ++t)
+                if (t == r) ai = a[ib + t];
+            const double alpha = Gr::lane_value(ai, i);
+            const double xn2 = Gr::sum_fixed(lane > i ? __fma_rn(ai, ai, 0.0) : 0.0);
+            double rii = ai;
+            if (i + 1 < m && xn2 != 0.0) {
+                const double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
+                const double beta = -copysign(nrm, alpha);
+                const double tau = (beta - alpha) / beta;
+                const double scal = 1.0 / (alpha - beta);
+                const double vi = lane > i ? ai * scal : (lane == i ? 1.0 : 0.0);
+                const double ni = lane > i ? vi : (lane == i ? beta : ai);
+                rii = beta;
 #pragma unroll
-            for (int jb = 0; jb <= NMAX; jb += kB) {
-                /* the block holds a slot in (i, n) or the right-hand side */
-                if ((jb + kB - 1 > i && jb < n) || jb + kB > NMAX) {
-                    double p[kB];
+                for (int t = 0; t < kB; ++t)
+                    if (t == r) a[ib + t] = ni;
+                const double mt = -tau;
+                const bool act = lane >= i;
 #pragma unroll
-                    for (int t = 0; t < kB; ++t)
-                        p[t] = (jb + t <= NMAX && act) ? __fma_rn(vi, a[jb + t <= NMAX ? jb + t : NMAX], 0.0) : 0.0;
-                    Gr::sum_fixed_n(p);
+                for (int jb = ib; jb <= NMAX; jb += kB) {
+                    /* the block holds a slot in (i, n) or the right-hand side */
+                    if ((jb + kB - 1 > i && jb < n) || jb + kB > NMAX) {
+                        double p[kB];
 #pragma unroll
-                    for (int t = 0; t < kB; ++t) {
-                        const int j = jb + t;
-                        if (j <= NMAX) {
-                            const bool live = j == NMAX || (j > i && j < n);
-                            if (live && act) a[j] = __fma_rn(vi, mt * p[t], a[j]);
+                        for (int t = 0; t < kB; ++t)
+                            p[t] = (jb + t <= NMAX && act) ? __fma_rn(vi, a[jb + t <= NMAX ? jb + t : NMAX], 0.0)
+                                                           : 0.0;
+                        Gr::sum_bcast_n(p);
+#pragma unroll
+                        for (int t = 0; t < kB; ++t) {
+                            const int j = jb + t;
+                            if (j <= NMAX) {
+                                const bool live = j == NMAX || (j > i && j < n);
+                                if (live && act) a[j] = __fma_rn(vi, mt * p[t], a[j]);
+                            }
                         }
                     }
                 }
             }
+            zero_diag |= lane == i && rii == 0.0;
         }
     }
     double b = a[NMAX];
@@ -561,25 +626,20 @@ __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveS
     if (m < n) {
         st = COL_UNDERDET;
         b = 0.0;
+    } else if (Gr::ballot(zero_diag) != 0ull) {
+        st = COL_SINGULAR;
     } else {
-        double d = 0.0;
+        /* dtrsm order; K runs over compile-time slots so R_iK is a fixed
+         * register of lane i */
 #pragma unroll
-        for (int t = 0; t < NMAX; ++t)
-            if (t == lane) d = a[t];
-        if (Gr::ballot(lane < n && d == 0.0) != 0ull) {
-            st = COL_SINGULAR;
-        } else {
-            for (int K = n - 1; K >= 0; --K) {
-                double rk = 0.0;
-#pragma unroll
-                for (int t = 0; t < NMAX; ++t)
-                    if (t == K) rk = a[t];
+        for (int K = NMAX - 1; K >= 0; --K) {
+            if (K < n) {
                 double xk = Gr::lane_value(b, K);
-                const double rkk = Gr::lane_value(rk, K);
+                const double rkk = Gr::lane_value(a[K], K);
                 if (xk != 0.0) {
                     xk = xk / rkk;
                     if (lane == K) b = xk;
-                    if (lane < K) b = __fma_rn(-xk, rk, b);
+                    if (lane < K) b = __fma_rn(-xk, a[K], b);
                 }
             }
         }
@@ -680,7 +740,15 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         }
         int incl = Gr::scan_incl(len);
         int excl = incl - len;
-        for (int e = 0; e < len; ++e) s.sbuf[off + excl + e] = a.rp_ind[e0 + e];
+        /* four loads in flight per round instead of one */
+        for (int e = 0; e < len; e += 4) {
+            int v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = e + q < len ? a.rp_ind[e0 + e + q] : 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (e + q < len) s.sbuf[off + excl + e + q] = v[q];
+        }
         off += __shfl(incl, G - 1, G);
     }
     __syncthreads();
@@ -711,10 +779,22 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         int j = s.jt[t];
         double num = 0.0;
         int e0 = a.A.colptr[j], e1 = a.A.colptr[j + 1];
-        for (int e = e0; e < e1; ++e) {
-            int row = a.A.rowind[e];
-            int p = lower_bound_i(s.I, m, row);
-            if (p < m && s.I[p] == row) num = __dadd_rn(num, __dmul_rn(s.r[p], a.A.val[e]));
+        /* entries in order, four loads in flight per round */
+        for (int e = e0; e < e1; e += 4) {
+            int rows[4];
+            double vals[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                rows[q] = e + q < e1 ? a.A.rowind[e + q] : 0;
+                vals[q] = e + q < e1 ? a.A.val[e + q] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (e + q < e1) {
+                    int p = lower_bound_i(s.I, m, rows[q]);
+                    if (p < m && s.I[p] == rows[q]) num = __dadd_rn(num, __dmul_rn(s.r[p], vals[q]));
+                }
+            }
         }
         num = __dmul_rn(num, num);
         double ajn = a.ajn[j];
@@ -843,80 +923,6 @@ __device__ __forceinline__ void count_solve(const BuildArgs& a, const WaveScratc
     wk[7] += dn;
 }
 
-/* v <- (R^T R)^-1 v in FP64 for the upper-triangular R stored column-major
- * in q (leading dimension ld; FP64 or FP32 entries promoted to FP64). */
-template <int G, typename RT>
-__device__ __forceinline__ void normal_solve(const RT* q, int ld, double* v, int n)
-{
-    using Gr = Group<G>;
-    const int lane = Gr::lane();
-    for (int K = 0; K < n; ++K) {  // R^T y = v
-        double yk = v[K] / (double)q[K * ld + K];
-        __syncthreads();
-        if (lane == 0) v[K] = yk;
-        for (int i = K + 1 + lane; i < n; i += G) v[i] = __fma_rn(-yk, (double)q[i * ld + K], v[i]);
-        __syncthreads();
-    }
-    for (int K = n - 1; K >= 0; --K) {  // R d = y
-        double dk = v[K] / (double)q[K * ld + K];
-        __syncthreads();
-        if (lane == 0) v[K] = dk;
-        for (int i = lane; i < K; i += G) v[i] = __fma_rn(-dk, (double)q[K * ld + i], v[i]);
-        __syncthreads();
-    }
-}
-
-/* out[i] = sum_j C(I_i, J_j) w_j over I (FP64, from the CSC columns); when
- * `sub` is set, out = base - C w instead. */
-template <int G>
-__device__ __forceinline__ void csc_apply(const BuildArgs& a, const WaveScratch& s, int m, int n, const double* w,
-                                          const double* base, double* out)
-{
-    using Gr = Group<G>;
-    const int lane = Gr::lane();
-    for (int i = lane; i < m; i += G) out[i] = base ? base[i] : 0.0;
-    __syncthreads();
-    const double sign = base ? -1.0 : 1.0;
-    for (int jj = 0; jj < n; ++jj) {
-        double wj = sign * w[jj];
-        int j = s.J[jj];
-        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-        for (int e = lane; e < cnt; e += G) {
-            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-            out[pos] = __fma_rn(wj, a.A.val[e0 + e], out[pos]);
-        }
-        __syncthreads();
-    }
-}
-
-/* out_j = sum_i C(I_i, J_j) v_i (FP64, one group reduction per column). */
-template <int G>
-__device__ __forceinline__ void csc_apply_t(const BuildArgs& a, const WaveScratch& s, int m, int n, const double* v,
-                                            double* out)
-{
-    using Gr = Group<G>;
-    const int lane = Gr::lane();
-    for (int jj = 0; jj < n; ++jj) {
-        int j = s.J[jj];
-        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-        double p = 0.0;
-        for (int e = lane; e < cnt; e += G) {
-            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-            p = __fma_rn(a.A.val[e0 + e], v[pos], p);
-        }
-        p = Gr::sum_fixed(p);
-        if (lane == 0) out[jj] = p;
-    }
-    __syncthreads();
-}
-
-/* Adjoint of one column of a build with fixed pattern J (M's column).  With
- * m = argmin ||C(I,J) m - b(I)|| and g = dL/dm:
- *   r = b - C m,  R^T R lambda = g,  s = C lambda,
- *   dL/dC(i, j) = r_i lambda_j - s_i m_j   (accumulated into gC, atomics),
- *   dL/db(i)    = s_i                      (written into gB).
- * The QR is recomputed (FP64 in the fixed order, or FP32 with refinement of
- * the normal-equation solve in mixed mode). */
 template <int G, bool kMixed>
 __device__ __forceinline__ int backward_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                                int col)
@@ -944,15 +950,15 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
         if (ls_solve<G>(L, s, m, n) & (COL_SINGULAR | COL_UNDERDET)) return kDone;
     }
     /* r = b - C m (into s.r) */
-    csc_apply<G>(a, s, m, n, s.y, s.b0, s.r);
+    csc_apply<G>(a, L, s, m, n, s.y, s.b0, s.r, kMixed);
     if (kMixed) {
         for (int i = lane; i < n; i += G) s.z[i] = s.g[i];
         __syncthreads();
         normal_solve<G>(s.aq32, ld, s.g, n);
         bool ok = false;
         for (int it = 0; it < kRefineSteps; ++it) {
-            csc_apply<G>(a, s, m, n, s.g, nullptr, s.b0);  // u = C lambda
-            csc_apply_t<G>(a, s, m, n, s.b0, s.b);          // C^T u
+            csc_apply<G>(a, L, s, m, n, s.g, nullptr, s.b0, true);  // u = C lambda
+            csc_apply_t<G>(a, L, s, m, n, s.b0, s.b, true);          // C^T u
             for (int i = lane; i < n; i += G) s.b[i] = s.z[i] - s.b[i];
             __syncthreads();
             normal_solve<G>(s.aq32, ld, s.b, n);
@@ -978,16 +984,15 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
         normal_solve<G>(s.aqr, ld, s.g, n);
     }
     /* s = C lambda (into s.b0) */
-    csc_apply<G>(a, s, m, n, s.g, nullptr, s.b0);
-    for (int jj = 0; jj < n; ++jj) {
-        double lj = s.g[jj], xj = s.y[jj];
-        int j = s.J[jj];
-        int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-        for (int e = lane; e < cnt; e += G) {
-            int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
-            double v = __fma_rn(s.r[pos], lj, -s.b0[pos] * xj);
-            atomicAdd(&a.gC[e0 + e], v);
-        }
+    csc_apply<G>(a, L, s, m, n, s.g, nullptr, s.b0, kMixed);
+    /* dL/dC_ij += r_i lambda_j - s_i m_j over the stored entries of J (the
+     * column segments build_I left) */
+    const int total = s.segoff[n];
+    for (int e = lane; e < total; e += G) {
+        const int t = s.sid[e];
+        const int src = s.segs[t] + e - s.segoff[t];
+        const int pos = lower_bound_i(s.I, m, a.A.rowind[src]);
+        atomicAdd(&a.gC[src], __fma_rn(s.r[pos], s.g[t], -s.b0[pos] * s.y[t]));
     }
     if (a.gB && a.B.colptr) {
         int e0 = a.B.colptr[col], cnt = a.B.colptr[col + 1] - e0;
@@ -1442,13 +1447,13 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         if (m == kOverflow) return kMoveOn;
         gather_ahat<G, kMixed>(a, L, s, m, n);
         st |= make_rhs<G>(a, s, m, col);
-        bool dense = false;
+        bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {
             if (ls_solve_mixed<G>(a, L, s, m, n) == kFallback) return kRedoFp64;
         } else if (a.cache_slots > 0) {
             st |= ls_solve_cached<G>(a, L, s, m, n);
         } else if (NMAX > 0 && m <= G && n <= NMAX) {
-            st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 1)>(L, s, m, n);
+            st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n);
             dense = true;
         } else {
             st |= ls_solve<G>(L, s, m, n);

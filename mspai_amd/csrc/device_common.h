@@ -314,30 +314,96 @@ struct Group {
         p = sum_step_f<1>(p);
         return p;
     }
+    /* 32-bit value of lane (l xor S), S < G, without LDS-pipe permutes:
+     * DPP for S < 16, a lane swap for 16 and 32 (of the two swap results the
+     * partner's value is the one from the other row / half) */
+    template <int S>
+    static __device__ __forceinline__ int partner(int v)
+    {
+        if (S < 16) return xor_lane<S>(v);
+        if (S == 16) {
+            auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+            return (threadIdx.x & 16) ? (int)r[0] : (int)r[1];
+        }
+        auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (threadIdx.x & 32) ? (int)r[0] : (int)r[1];
+    }
+    template <int S>
+    static __device__ __forceinline__ double partner(double v)
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = partner<S>((int)b), hi = partner<S>((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+    template <int S>
+    static __device__ __forceinline__ float partner(float v)
+    {
+        return __int_as_float(partner<S>(__float_as_int(v)));
+    }
+    /* all-reduce with an exact, order-free operation (min, max, integer add) */
+    template <class T, class Op>
+    static __device__ __forceinline__ T allreduce(T p, Op op)
+    {
+        if (G > 32) p = op(p, partner<32>(p));
+        if (G > 16) p = op(p, partner<16>(p));
+        p = op(p, partner<8>(p));
+        p = op(p, partner<4>(p));
+        p = op(p, partner<2>(p));
+        p = op(p, partner<1>(p));
+        return p;
+    }
     static __device__ __forceinline__ double min(double p)
     {
-#pragma unroll
-        for (int s = G / 2; s >= 1; s >>= 1) p = fmin(p, __shfl_xor(p, s, G));
-        return p;
+        return allreduce(p, [](double x, double y) { return fmin(x, y); });
+    }
+    static __device__ __forceinline__ double max(double p)
+    {
+        return allreduce(p, [](double x, double y) { return fmax(x, y); });
+    }
+    static __device__ __forceinline__ float minf(float p)
+    {
+        return allreduce(p, [](float x, float y) { return fminf(x, y); });
+    }
+    static __device__ __forceinline__ float maxf(float p)
+    {
+        return allreduce(p, [](float x, float y) { return fmaxf(x, y); });
     }
     static __device__ __forceinline__ int sum_int(int p)
     {
-#pragma unroll
-        for (int s = G / 2; s >= 1; s >>= 1) p += __shfl_xor(p, s, G);
-        return p;
+        return allreduce(p, [](int x, int y) { return x + y; });
+    }
+    /* 32-bit value of group lane `src` (a compile-time lane): readlane for
+     * a whole wave, ds_swizzle's broadcast mode inside 16/32-lane groups */
+    template <int SRC>
+    static __device__ __forceinline__ int lane_bcast(int v)
+    {
+        static_assert(SRC >= 0 && SRC < G, "source lane");
+        if (G == kWave) return __builtin_amdgcn_readlane(v, SRC);
+        /* lane l reads ((l & and) | or) within its 32-lane half */
+        return __builtin_amdgcn_ds_swizzle(v, (G == 16 ? 0x10 : 0x00) | (SRC << 5));
+    }
+    template <int SRC>
+    static __device__ __forceinline__ double lane_bcast(double v)
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = lane_bcast<SRC>((int)b), hi = lane_bcast<SRC>((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     }
     /* value of lane 0 of the group */
-    template <class T>
-    static __device__ __forceinline__ T bcast0(T v) { return __shfl(v, 0, G); }
-    /* inclusive prefix sum over the group */
+    static __device__ __forceinline__ int bcast0(int v) { return lane_bcast<0>(v); }
+    static __device__ __forceinline__ double bcast0(double v) { return lane_bcast<0>(v); }
+    /* value of the group's last lane */
+    static __device__ __forceinline__ int bcast_last(int v) { return lane_bcast<G - 1>(v); }
+    /* inclusive prefix sum over the group: DPP row shifts inside each row of
+     * 16 lanes, then row_bcast:15 / row_bcast:31 carry the row totals on */
     static __device__ __forceinline__ int scan_incl(int v)
     {
-        const int l = lane();
-#pragma unroll
-        for (int d = 1; d < G; d <<= 1) {
-            int u = __shfl_up(v, d, G);
-            if (l >= d) v += u;
-        }
+        v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+        v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+        v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+        v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+        if (G > 16) v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // rows 1,3 += lane 15, 47
+        if (G > 32) v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // rows 2,3 += lane 31
         return v;
     }
     /* ballot restricted to the group, bit i = group lane i */

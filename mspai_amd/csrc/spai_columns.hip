@@ -145,7 +145,7 @@ __device__ __forceinline__ int load_segments(const BuildArgs& a, const WaveScrat
             s.segs[t] = start;
             s.segoff[t] = carry + incl - cnt;
         }
-        carry += __shfl(incl, G - 1, G);
+        carry += Gr::bcast_last(incl);
     }
     if (lane == 0) s.segoff[n] = carry;
     __syncthreads();
@@ -485,10 +485,8 @@ __device__ __forceinline__ int qr_f32(const ScratchLayout& L, const WaveScratch&
         dmax = fmaxf(dmax, d);
         dmin = fminf(dmin, d);
     }
-    for (int o = G / 2; o >= 1; o >>= 1) {
-        dmax = fmaxf(dmax, __shfl_xor(dmax, o, G));
-        dmin = fminf(dmin, __shfl_xor(dmin, o, G));
-    }
+    dmax = Gr::maxf(dmax);
+    dmin = Gr::minf(dmin);
     return (dmin > kCondGuard * dmax) ? 0 : kFallback;
 }
 
@@ -522,10 +520,8 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
             dmx = fmax(dmx, fabs(s.g[i]));
             xmx = fmax(xmx, fabs(x));
         }
-        for (int o = G / 2; o >= 1; o >>= 1) {
-            dmx = fmax(dmx, __shfl_xor(dmx, o, G));
-            xmx = fmax(xmx, __shfl_xor(xmx, o, G));
-        }
+        dmx = Gr::max(dmx);
+        xmx = Gr::max(xmx);
         __syncthreads();
         /* each step shrinks the error by about kappa^2 u_32; a correction
          * below 1e-10 |x| leaves the next one below FP64 rounding */
@@ -749,7 +745,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
             for (int q = 0; q < 4; ++q)
                 if (e + q < len) s.sbuf[off + excl + e + q] = v[q];
         }
-        off += __shfl(incl, G - 1, G);
+        off += Gr::bcast_last(incl);
     }
     __syncthreads();
     int nu = group_sort_unique<G>(s.sbuf, off, s.sbuf);
@@ -969,10 +965,8 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
                 dmx = fmax(dmx, fabs(s.b[i]));
                 lmx = fmax(lmx, fabs(l));
             }
-            for (int o = G / 2; o >= 1; o >>= 1) {
-                dmx = fmax(dmx, __shfl_xor(dmx, o, G));
-                lmx = fmax(lmx, __shfl_xor(lmx, o, G));
-            }
+            dmx = Gr::max(dmx);
+            lmx = Gr::max(lmx);
             __syncthreads();
             if (dmx <= 1e-10 * lmx) {
                 ok = true;
@@ -1307,8 +1301,7 @@ __device__ __forceinline__ uint32_t cache_key(const double* A, int ld, int m, in
         int j = t / m, i = t - j * m;
         acc += (mix + hash_double(A[j * ld + i])) * pow31(N - 1u - (uint32_t)t);
     }
-    for (int o = G / 2; o >= 1; o >>= 1) acc += (uint32_t)__shfl_xor((int)acc, o, G);
-    return acc;
+    return (uint32_t)Gr::allreduce((int)acc, [](int x, int y) { return (int)((uint32_t)x + (uint32_t)y); });
 }
 
 /* doubles per slot: A, factor (m x n each), tau, x (n), b (m) */
@@ -1493,7 +1486,7 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
     if (mask == 0ull) return;
     int base = 0;
     if (lane_id() == 0) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, 0, kWave);
+    base = __builtin_amdgcn_readlane(base, 0);
     if (flag) out[base + __popcll(mask & ((1ull << lane_id()) - 1ull))] = col;
 }
 
@@ -1521,7 +1514,7 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
             q0 = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (q0 < nq) q0 = atomicAdd(head, NG * kRounds);
         }
-        q0 = __shfl(q0, 0, kWave);
+        q0 = __builtin_amdgcn_readlane(q0, 0);
         if (q0 >= nq) break;
         for (int round = 0; round < kRounds; ++round) {
             int q = q0 + round * NG + g;

@@ -490,25 +490,122 @@ __device__ __forceinline__ int qr_f32(const ScratchLayout& L, const WaveScratch&
     return (dmin > kCondGuard * dmax) ? 0 : kFallback;
 }
 
-template <int G>
+/* FP32 Householder QR of A(I,J) with row r in the registers of group lane r
+ * (|I| <= G, |J| <= NMAX), the FP32 right-hand side in slot NMAX; x0 =
+ * R^-1 Q^T b in FP32.  Leaves R (upper triangle, column-major) in s.aq32 for
+ * the refinement and x0 in s.b.  kFallback when min |R_ii| <= kCondGuard
+ * max |R_ii| (summation order is free here: the FP64 refinement decides). */
+template <int G, int NMAX>
+__device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScratch& s, int m, int n)
+{
+    using Gr = Group<G>;
+    constexpr int kB = 4;
+    const int lane = Gr::lane();
+    const int ld = L.capI;
+    if (m < n || n == 0) return kFallback;
+    const bool row = lane < m;
+    float a[NMAX + 1];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? s.aq32[j * ld + lane] : 0.f;
+    a[NMAX] = row ? (float)s.b[lane] : 0.f;
+    float dmax = 0.f, dmin = INFINITY;
+    const auto plus = [](float x, float y) { return x + y; };
+#pragma unroll
+    for (int ib = 0; ib < NMAX; ib += kB) {
+        if (ib >= n) break;
+#pragma unroll 1
+        for (int r = 0; r < kB; ++r) {
+            const int i = ib + r;
+            if (i >= n) break;
+            float ai = a[ib];
+#pragma unroll
+            for (int t = 1; t < kB; ++t)
+                if (t == r) ai = a[ib + t];
+            const float alpha = __shfl(ai, i, G);
+            const float xn2 = Gr::allreduce(lane > i ? ai * ai : 0.f, plus);
+            float rii = ai;
+            if (i + 1 < m && xn2 != 0.f) {
+                const float nrm = sqrtf(__fmaf_rn(alpha, alpha, xn2));
+                const float beta = -copysignf(nrm, alpha);
+                const float tau = (beta - alpha) / beta;
+                const float scal = 1.f / (alpha - beta);
+                const float vi = lane > i ? ai * scal : (lane == i ? 1.f : 0.f);
+                const float ni = lane > i ? vi : (lane == i ? beta : ai);
+                rii = beta;
+#pragma unroll
+                for (int t = 0; t < kB; ++t)
+                    if (t == r) a[ib + t] = ni;
+                const bool act = lane >= i;
+#pragma unroll
+                for (int jb = ib; jb <= NMAX; jb += kB) {
+                    if ((jb + kB - 1 > i && jb < n) || jb + kB > NMAX) {
+                        float w[kB];
+#pragma unroll
+                        for (int t = 0; t < kB; ++t)
+                            w[t] = (jb + t <= NMAX && act) ? vi * a[jb + t <= NMAX ? jb + t : NMAX] : 0.f;
+#pragma unroll
+                        for (int t = 0; t < kB; ++t)
+                            if (jb + t <= NMAX) w[t] = Gr::allreduce(w[t], plus);
+#pragma unroll
+                        for (int t = 0; t < kB; ++t) {
+                            const int j = jb + t;
+                            if (j <= NMAX) {
+                                const bool live = j == NMAX || (j > i && j < n);
+                                if (live && act) a[j] = __fmaf_rn(vi, -tau * w[t], a[j]);
+                            }
+                        }
+                    }
+                }
+            }
+            if (lane == i) {
+                dmax = fmaxf(dmax, fabsf(rii));
+                dmin = fminf(dmin, fabsf(rii));
+            }
+        }
+    }
+    dmax = Gr::maxf(dmax);
+    dmin = Gr::minf(dmin);
+    if (!(dmin > kCondGuard * dmax)) return kFallback;
+    float x = a[NMAX];
+#pragma unroll
+    for (int K = NMAX - 1; K >= 0; --K) {
+        if (K < n) {
+            float xk = __shfl(x, K, G) / __shfl(a[K], K, G);
+            if (lane == K) x = xk;
+            if (lane < K) x = __fmaf_rn(-xk, a[K], x);
+        }
+    }
+#pragma unroll
+    for (int K = 0; K < NMAX; ++K)
+        if (K < n && lane <= K) s.aq32[K * ld + lane] = a[K];
+    if (lane < n) s.b[lane] = (double)x;
+    __syncthreads();
+    return 0;
+}
+
+template <int G, int NMAX>
 __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                               int m, int n)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
     const int ld = L.capI;
-    if (qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
     const float* q = s.aq32;
-    /* x0 in FP32, column-oriented back substitution */
-    for (int K = n - 1; K >= 0; --K) {
-        float xk = s.bq32[K] / q[K * ld + K];
-        __syncthreads();
-        if (lane == 0) s.bq32[K] = xk;
-        for (int i = lane; i < K; i += G) s.bq32[i] = __fmaf_rn(-xk, q[K * ld + i], s.bq32[i]);
+    if (NMAX > 0 && m <= G && n <= NMAX) {
+        if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kFallback;
+    } else {
+        if (qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
+        /* x0 in FP32, column-oriented back substitution */
+        for (int K = n - 1; K >= 0; --K) {
+            float xk = s.bq32[K] / q[K * ld + K];
+            __syncthreads();
+            if (lane == 0) s.bq32[K] = xk;
+            for (int i = lane; i < K; i += G) s.bq32[i] = __fmaf_rn(-xk, q[K * ld + i], s.bq32[i]);
+            __syncthreads();
+        }
+        for (int i = lane; i < n; i += G) s.b[i] = (double)s.bq32[i];
         __syncthreads();
     }
-    for (int i = lane; i < n; i += G) s.b[i] = (double)s.bq32[i];
-    __syncthreads();
     for (int it = 0; it < kRefineSteps; ++it) {
         csc_apply<G>(a, L, s, m, n, s.b, s.b0, s.r, true);  // r = b0 - A x (FP64)
         csc_apply_t<G>(a, L, s, m, n, s.r, s.g, true);      // g = A^T r
@@ -1442,7 +1539,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         st |= make_rhs<G>(a, s, m, col);
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {
-            if (ls_solve_mixed<G>(a, L, s, m, n) == kFallback) return kRedoFp64;
+            if (ls_solve_mixed<G, NMAX>(a, L, s, m, n) == kFallback) return kRedoFp64;
         } else if (a.cache_slots > 0) {
             st |= ls_solve_cached<G>(a, L, s, m, n);
         } else if (NMAX > 0 && m <= G && n <= NMAX) {
@@ -1573,12 +1670,13 @@ __global__ void copy_ints_kernel(const int* src, int* dst, int count)
 
 using ColumnKernel = void (*)(BuildArgs, ScratchLayout, const int*, const int*, int*);
 
-/* The LDS column kernel for a bin: register-resident QR (FP64 build) when a
- * group holds every row of the bin (|I| <= G) and the bin's |J| fits the
- * unrolled width. */
+/* The LDS column kernel for a bin: register-resident QR (FP64 or mixed
+ * build) when a group holds every row of the bin (|I| <= G) and the bin's
+ * |J| fits the unrolled width. */
 static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
 {
-    const bool regs = mode == 0 && L.capI <= group && L.capJ <= (group == 16 ? 8 : (group == 32 ? 16 : 32));
+    const bool regs = (mode == 0 || mode == 1) && L.capI <= group &&
+                      L.capJ <= (group == 16 ? 8 : (group == 32 ? 16 : 32));
 #define MSPAI_PICK(MODE)                                                                            \
     if (group == 16) return &spai_column_kernel<16, true, MODE, 0>;                                 \
     if (group == 32) return &spai_column_kernel<32, true, MODE, 0>;                                 \
@@ -1588,7 +1686,10 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
         if (group == 16) return regs ? &spai_column_kernel<16, true, 0, 8> : &spai_column_kernel<16, true, 0, 0>;
         if (group == 32) return regs ? &spai_column_kernel<32, true, 0, 16> : &spai_column_kernel<32, true, 0, 0>;
         return regs ? &spai_column_kernel<64, true, 0, 32> : &spai_column_kernel<64, true, 0, 0>;
-    case 1: MSPAI_PICK(1);
+    case 1:
+        if (group == 16) return regs ? &spai_column_kernel<16, true, 1, 8> : &spai_column_kernel<16, true, 1, 0>;
+        if (group == 32) return regs ? &spai_column_kernel<32, true, 1, 16> : &spai_column_kernel<32, true, 1, 0>;
+        return regs ? &spai_column_kernel<64, true, 1, 32> : &spai_column_kernel<64, true, 1, 0>;
     case 2: MSPAI_PICK(2);
     case 3: MSPAI_PICK(3);
     default: MSPAI_PICK(4);

@@ -620,6 +620,8 @@ static const BinTable kTables[] = {
     {"g16-g32-g64-g64/occ12", 4,
      {{16, 8, 16, 64, 128, 12}, {32, 16, 32, 128, 256, 12}, {64, 32, 64, 256, 512, 8},
       {64, 40, 128, 512, 1024, 4}}},
+    {"g16-g64/occ8", 2, {{16, 8, 16, 64, 128, 8}, {64, 40, 128, 512, 1024, 0}}},
+    {"g32-g64-g64/occ8", 3, {{32, 16, 32, 128, 256, 8}, {64, 32, 64, 256, 512, 0}, {64, 40, 128, 512, 1024, 0}}},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 
@@ -669,6 +671,20 @@ static int waves_per_cu(const Bin& b, int mode)
     return w;
 }
 
+/* The layout with its chunk size for `expected` columns.  Every resident
+ * wave starts on one chunk by its block index; when the columns fit in one
+ * chunk per wave (at most 16 columns per group), that static split is all
+ * (no queue atomics), else chunks of 16 columns per group follow through the
+ * queue atomic (all waves take the same address, so fewer, larger chunks beat
+ * finer balance).  A bin that gets only overflow or fallback columns takes
+ * one column per group. */
+static ScratchLayout with_rounds(ScratchLayout L, long long expected, int nwaves, int group)
+{
+    const long long groups = std::max(1LL, (long long)nwaves * (64 / group));
+    L.rounds = (int)std::max(1LL, std::min(16LL, (expected + groups - 1) / groups));
+    return L;
+}
+
 /* Launch routing and the bin chain over the columns whose patterns are in
  * a.p_ptr/p_ind.  Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb =
  * generic), mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills
@@ -714,11 +730,11 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         a.fallback = fq;
         a.n_fallback = counts;
         for (int b = 0; b < nb; ++b) {
-            const ScratchLayout& L = bins[b].L;
             const int grp = bins[b].group;
             a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
             a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
             int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
+            const ScratchLayout L = with_rounds(bins[b].L, b == 0 ? n : 0, ctx->num_cu * per_cu, grp);
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
                           ctx->num_cu * per_cu, ctx->stream);
             HIPCHK(hipGetLastError());
@@ -732,11 +748,13 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     a.fallback = nullptr;
     a.n_fallback = nullptr;
     for (int b = 0; b < nb; ++b) {
-        const ScratchLayout& L = bins[b].L;
         const int grp = bins[b].group;
         a.overflow = fq + (size_t)(b + 1) * n;
         a.n_overflow = counts + b + 1;
         int per_cu = waves_per_cu(bins[b], fp64_mode);
+        /* the first FP64 bin holds the routed columns unless the mixed chain
+         * ran first (then only its fallbacks arrive) */
+        const ScratchLayout L = with_rounds(bins[b].L, (b == 0 && !mixed) ? n : 0, ctx->num_cu * per_cu, grp);
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());

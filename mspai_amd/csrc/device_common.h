@@ -48,6 +48,7 @@ struct ScratchLayout {
     int capSort;   // raw index lists before sorting (a power of two)
     int capPairs;  // candidates on the std::sort tie path
     int tie_in_main;        // pairs/stack in the main region (else in BuildArgs::tie_scratch)
+    int rounds;             // columns per group in one chunk of the work queue
     size_t tie_per_group;   // bytes of a group's tie region when not in the main one
 };
 
@@ -438,6 +439,49 @@ __device__ __forceinline__ int lower_bound_i(const int* a, int n, int key)
  * padded with INT_MAX; buf must hold that many ints) and write the distinct
  * values to out (out may equal buf).  Whole-group call; the wave must be
  * converged at the barriers.  Returns the number of distinct values. */
+/* Sorted distinct values of buf[0..count) into out (out may be buf).
+ * For a whole wave (G = 64) and up to 8 values per lane the bitonic network
+ * runs in registers: element i = k*64 + lane sits in slot k of its lane;
+ * partners at distance j < 64 are reached with DPP / lane swaps, larger
+ * distances pair slots of the same lane.  Otherwise (16/32-lane groups, where
+ * the LDS network measured faster, or longer lists) the same network runs
+ * through buf in LDS.  Either way the result is written back to buf and
+ * compacted by ballot. */
+template <int G, int J>
+__device__ __forceinline__ void bitonic_lanes(int (&x)[8], int kslots, int size)
+{
+    const int l = Group<G>::lane();
+    const bool low = (l & J) == 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k < kslots) {
+            const int i = k * G + l;
+            const bool up = (i & size) == 0;
+            const int p = Group<G>::template partner<J>(x[k]);
+            x[k] = (low == up) ? min(x[k], p) : max(x[k], p);
+        }
+    }
+}
+
+template <int G>
+__device__ __forceinline__ void bitonic_slots(int (&x)[8], int kslots, int size, int js)
+{
+    const int l = Group<G>::lane();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {
+            if (d == js && (k & d) == 0 && (k | d) < kslots) {
+                const int i = k * G + l;
+                const bool up = (i & size) == 0;
+                const int a = x[k], b = x[k | d];
+                x[k] = up ? min(a, b) : max(a, b);
+                x[k | d] = up ? max(a, b) : min(a, b);
+            }
+        }
+    }
+}
+
 template <int G>
 __device__ __forceinline__ int group_sort_unique(int* buf, int count, int* out)
 {
@@ -445,24 +489,52 @@ __device__ __forceinline__ int group_sort_unique(int* buf, int count, int* out)
     const int l = Gr::lane();
     int N = 1;
     while (N < count) N <<= 1;
-    for (int i = count + l; i < N; i += G) buf[i] = 0x7fffffff;
-    __syncthreads();
-    for (int k = 2; k <= N; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = l; i < N; i += G) {
-                int p = i ^ j;
-                if (p > i) {
-                    int a = buf[i], b = buf[p];
-                    bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        buf[i] = b;
-                        buf[p] = a;
+    if (G == kWave && N <= 8 * G) {
+        const int kslots = (N + G - 1) / G;
+        int x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = (k < kslots && k * G + l < count) ? buf[k * G + l] : 0x7fffffff;
+        for (int size = 2; size <= N; size <<= 1) {
+            for (int j = size >> 1; j > 0; j >>= 1) {
+                if (j >= G) {
+                    bitonic_slots<G>(x, kslots, size, j / G);
+                } else {
+                    switch (j) {
+                    case 1: bitonic_lanes<G, 1>(x, kslots, size); break;
+                    case 2: bitonic_lanes<G, 2>(x, kslots, size); break;
+                    case 4: bitonic_lanes<G, 4>(x, kslots, size); break;
+                    case 8: bitonic_lanes<G, 8>(x, kslots, size); break;
+                    case 16: if (G > 16) bitonic_lanes<G, (G > 16 ? 16 : 1)>(x, kslots, size); break;
+                    default: if (G > 32) bitonic_lanes<G, (G > 32 ? 32 : 1)>(x, kslots, size); break;
                     }
                 }
             }
-            __syncthreads();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < kslots && k * G + l < count) buf[k * G + l] = x[k];
+    } else {
+        for (int i = count + l; i < N; i += G) buf[i] = 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= N; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = l; i < N; i += G) {
+                    int p = i ^ j;
+                    if (p > i) {
+                        int a = buf[i], b = buf[p];
+                        bool up = (i & k) == 0;
+                        if ((a > b) == up) {
+                            buf[i] = b;
+                            buf[p] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
         }
     }
+    __syncthreads();
     int total = 0;
     for (int base = 0; base < count; base += G) {
         int i = base + l;

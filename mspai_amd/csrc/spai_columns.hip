@@ -105,6 +105,7 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const S
 }
 
 constexpr int kOverflow = -2;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, 32 CUs each
 constexpr int kFallback = -3;
 
 /* Outcome of one column in one launch. */
@@ -1595,26 +1596,40 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
 {
     extern __shared__ __align__(16) char smem[];
     constexpr int NG = kWave / G;
-    /* columns taken per queue atomic: one per group, several rounds */
-    constexpr int kRounds = G == 16 ? 8 : (G == 32 ? 4 : 2);
+    /* columns taken per chunk: one per group, L.rounds rounds; a wave's first
+     * chunk is its block index, later chunks come from the queue atomic */
+    const int rounds = L.rounds > 0 ? L.rounds : 1;
+    const int chunk = NG * rounds;
     const int g = Group<G>::index();
     char* base = kLds ? smem + (size_t)g * L.per_wave
                       : a.scratch + ((size_t)blockIdx.x * NG + g) * L.per_wave;
     char* tie = L.tie_in_main ? base : a.tie_scratch + ((size_t)blockIdx.x * NG + g) * L.tie_per_group;
     WaveScratch s = wave_scratch(base, tie, L);
     const int nq = *nqueue;
-    while (true) {
-        /* a plain load first: once the queue is drained (or empty) the waves
-         * leave without queueing on the atomic */
-        int q0 = 0;
-        if (lane_id() == 0) {
-            q0 = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (q0 < nq) q0 = atomicAdd(head, NG * kRounds);
+    const long long first = (long long)gridDim.x * chunk;  // chunks handed out by block index
+    for (long long pass = 0;; ++pass) {
+        long long q0 = 0;
+        if (pass == 0) {
+            /* workgroups go round-robin to the 8 XCDs: give XCD x the x-th
+             * contiguous eighth of the first chunks, so that neighbouring
+             * columns (which share rows of A) meet in the same L2 */
+            const int per_xcd = (int)(gridDim.x / kXcds);
+            const int b = blockIdx.x;
+            const long long slot = b < per_xcd * kXcds ? (long long)(b % kXcds) * per_xcd + b / kXcds : b;
+            q0 = slot * chunk;
+        } else {
+            /* a plain load first: once the queue is drained the waves leave
+             * without queueing on the atomic */
+            int h = 0;
+            if (lane_id() == 0) {
+                h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (first + h < nq) h = atomicAdd(head, chunk);
+            }
+            q0 = first + __builtin_amdgcn_readlane(h, 0);
         }
-        q0 = __builtin_amdgcn_readlane(q0, 0);
         if (q0 >= nq) break;
-        for (int round = 0; round < kRounds; ++round) {
-            int q = q0 + round * NG + g;
+        for (int round = 0; round < rounds; ++round) {
+            const long long q = q0 + round * NG + g;
             int col = -1;
             int outcome = kDone;
             if (q < nq) {

@@ -202,10 +202,12 @@ def test_cache_is_bitwise_neutral(oracle, gpu, make):
     M, info, stats = gpu.spai(prob, cache=4096)
     assert_bitwise(oracle, prob, ref, M, info)
     print(f"[cache {prob.name}] hits {stats['n_cache_hits']} of {int(info['nsolve'].sum())} solves", flush=True)
-    # hits depend on which group publishes a block first (a timing matter);
-    # the structured grids repeat most blocks, so a good share must hit
-    if prob.name in ("laplace2d_static", "convdiff2d_adaptive", "laplace3d_probing"):
-        assert stats["n_cache_hits"] > 0.25 * info["nsolve"].sum()
+    # a hit needs an identical block published by another group before the
+    # lookup, so the count depends on timing: a small static problem runs all
+    # its columns at once and may hit nothing, while the adaptive 2D grid
+    # (second solves of 65,536 columns) must reuse most of its blocks
+    if prob.name == "convdiff2d_adaptive":
+        assert stats["n_cache_hits"] > 0.5 * info["nsolve"].sum()
 
 
 def test_autotuned_bins_bitwise(oracle, gpu):

@@ -70,29 +70,37 @@ def _column_rel_err(M, Mref):
     return float(np.max(num / np.maximum(den, 1e-300)))
 
 
-def _bwd_fwd(mspai_amd, gen, torch):
-    """Backward/forward device time of the adjoint on the MSPAI probing
-    workload (3D Laplacian 64^3, static pattern(A), rho = 100)."""
-    prob = gen.config4()
+def _bwd_fwd(mspai_amd, gen, torch, prob=None, steps=4):
+    """Backward/forward device time of the adjoint on a static workload (by
+    default the MSPAI probing workload: 3D Laplacian 64^3, static pattern(A),
+    rho = 100), both directions on the autotuned bin table."""
+    name = None
+    if prob is None:
+        prob = gen.config4()
+        name = "MSPAI probing, 3D Laplacian 64^3 (n=262144), static P(A), rho=100"
     b = mspai_amd.Builder(0)
     b.upload(prob.A, prob.row_pattern())
     b.set_pattern(prob.P)
-    b.set_target(prob.B)
-    out = {"workload": "MSPAI probing, 3D Laplacian 64^3 (n=262144), static P(A), rho=100"}
-    for name, precision in (("fp64", 0), ("mixed", 1)):
+    if prob.B is not None:
+        b.set_target(prob.B)
+    out = {"workload": name or prob.name}
+    tune = b.autotune(repeats=1, max_impr_steps=0)
+    out["bins"] = tune["name"]
+    for label, precision in (("fp64", 0), ("mixed", 1)):
         fw, bw = [], []
-        for it in range(4):
-            st = b.build(max_impr_steps=0, precision=precision)
+        for it in range(steps):
+            st = b.build(max_impr_steps=0, precision=precision, binning=2)
             gm = torch.ones(st["nnz"], dtype=torch.float64, device="cuda")
             gc = torch.empty(prob.A.nnz, dtype=torch.float64, device="cuda")
-            gb = torch.empty(prob.B.nnz, dtype=torch.float64, device="cuda")
+            gb = torch.empty(prob.B.nnz, dtype=torch.float64, device="cuda") if prob.B is not None else None
             torch.cuda.synchronize()
-            sb = b.backward_device(gm.data_ptr(), gc.data_ptr(), gb.data_ptr(), precision=precision)
+            sb = b.backward_device(gm.data_ptr(), gc.data_ptr(), gb.data_ptr() if gb is not None else None,
+                                   precision=precision, binning=2)
             if it:
                 fw.append(st["build_ms"])
                 bw.append(sb["build_ms"])
-        out[name] = {"fwd_ms": float(np.mean(fw)), "bwd_ms": float(np.mean(bw)),
-                     "bwd_over_fwd": float(np.mean(bw) / np.mean(fw))}
+        out[label] = {"fwd_ms": float(np.mean(fw)), "bwd_ms": float(np.mean(bw)),
+                      "bwd_over_fwd": float(np.mean(bw) / np.mean(fw))}
     b.close()
     return out
 
@@ -184,8 +192,12 @@ def main():
                      "cache_hits_per_build": stc["n_cache_hits"],
                      "pattern_equal_fp64": bool(np.array_equal(b.download().indices, r64["M"].indices))}
     b.close()
-    bwd = (_bwd_fwd(mspai_amd, gen, torch)
-           if (torch is not None and args.precision == "both" and args.workload == "convdiff2d") else None)
+    bwd = None
+    if torch is not None and args.precision == "both":
+        if args.workload == "convdiff2d":
+            bwd = _bwd_fwd(mspai_amd, gen, torch)
+        elif args.workload in ("varcoef3d", "probing3d", "laplace2d"):
+            bwd = _bwd_fwd(mspai_amd, gen, torch, prob, steps=3)
     main_run = rmx if rmx is not None else r64
     M = main_run["M"]
     ms_per_step = main_run["ms_per_step"]

@@ -540,6 +540,7 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.sid = take(capSort * 4);
     L.wsum = take(8 * 8);
     if (needs & kNeedMixed) L.aq32 = take(capI * capJ * 4);
+    if (needs & kNeedBackward) L.rq = take((size_t)capJ * capJ * 8);
     if (needs & (kNeedMixed | kNeedBackward)) {
         L.bq32 = take(capB * 4);
         L.g = take((size_t)capJ * 8);

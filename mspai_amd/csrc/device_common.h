@@ -40,7 +40,7 @@ struct DevCsc {
 struct ScratchLayout {
     size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y, z;
     size_t srow, spos, iold, sposold, jold, colid, reflrows, bperm, yq;
-    size_t segs, segoff, sid, wsum, aq32;
+    size_t segs, segoff, sid, wsum, aq32, rq;
     size_t per_wave;
     int capJ;      // |J|
     int capI;      // |I| (also the leading dimension of A(I,J))

@@ -65,6 +65,7 @@ struct WaveScratch {
     int* segoff;   // first flat entry of column J[t]; segoff[|J|] = total
     int* sid;      // flat entry -> position t in J
     double* wsum;  // the column's work counters
+    double* rq;    // adjoint: R of the register QR (capJ x capJ)
 };
 
 __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const ScratchLayout& L)
@@ -101,6 +102,7 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const S
     s.segoff = (int*)(ws + L.segoff);
     s.sid = (int*)(ws + L.sid);
     s.wsum = (double*)(ws + L.wsum);
+    s.rq = (double*)(ws + L.rq);
     return s;
 }
 
@@ -647,7 +649,7 @@ __device__ __forceinline__ double row_pick(const double (&a)[NMAX], int j)
  * holds the right-hand side; a reflector is applied to four slots at a time
  * so that their reductions interleave (slots outside (i, n) and b keep their
  * values). */
-template <int G, int NMAX>
+template <int G, int NMAX, bool kROut = false>
 __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
     using Gr = Group<G>;
@@ -714,6 +716,16 @@ __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveS
             }
             zero_diag |= lane == i && rii == 0.0;
         }
+    }
+    if (kROut) {
+        /* the adjoint's normal-equation solves need R, not x: R_iK (i <= K)
+         * to s.rq, column-major with leading dimension capJ */
+#pragma unroll
+        for (int K = 0; K < NMAX; ++K)
+            if (K < n && lane <= K) s.rq[K * L.capJ + lane] = a[K];
+        __syncthreads();
+        if (m < n) return COL_UNDERDET;
+        return Gr::ballot(zero_diag) != 0ull ? COL_SINGULAR : 0;
     }
     double b = a[NMAX];
     int st = 0;
@@ -1017,7 +1029,7 @@ __device__ __forceinline__ void count_solve(const BuildArgs& a, const WaveScratc
     wk[7] += dn;
 }
 
-template <int G, bool kMixed>
+template <int G, bool kMixed, int NMAX>
 __device__ __forceinline__ int backward_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                                int col)
 {
@@ -1038,13 +1050,26 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
     if (m < n) return kDone;  // degenerate column: no gradient
     gather_ahat<G, kMixed>(a, L, s, m, n);
     make_rhs<G>(a, s, m, col);
+    /* the register QRs leave A(I,J) intact in LDS: products from it */
+    const bool regs = NMAX > 0 && m <= G && n <= NMAX;
+    const double* R = s.aqr;
+    int ldr = ld;
     if (kMixed) {
-        if (qr_f32<G>(L, s, m, n) == kFallback) return kRedoFp64;
+        if (regs) {
+            if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kRedoFp64;
+        } else if (qr_f32<G>(L, s, m, n) == kFallback) {
+            return kRedoFp64;
+        }
+    } else if (regs) {
+        if (ls_solve_regs<G, (NMAX > 0 ? NMAX : 4), true>(L, s, m, n) & (COL_SINGULAR | COL_UNDERDET)) return kDone;
+        R = s.rq;
+        ldr = L.capJ;
     } else {
         if (ls_solve<G>(L, s, m, n) & (COL_SINGULAR | COL_UNDERDET)) return kDone;
     }
+    const bool dense = kMixed || regs;
     /* r = b - C m (into s.r) */
-    csc_apply<G>(a, L, s, m, n, s.y, s.b0, s.r, kMixed);
+    csc_apply<G>(a, L, s, m, n, s.y, s.b0, s.r, dense);
     if (kMixed) {
         for (int i = lane; i < n; i += G) s.z[i] = s.g[i];
         __syncthreads();
@@ -1073,10 +1098,10 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
         }
         if (!ok) return kRedoFp64;
     } else {
-        normal_solve<G>(s.aqr, ld, s.g, n);
+        normal_solve<G>(R, ldr, s.g, n);
     }
     /* s = C lambda (into s.b0) */
-    csc_apply<G>(a, L, s, m, n, s.g, nullptr, s.b0, kMixed);
+    csc_apply<G>(a, L, s, m, n, s.g, nullptr, s.b0, dense);
     /* dL/dC_ij += r_i lambda_j - s_i m_j over the stored entries of J (the
      * column segments build_I left) */
     const int total = s.segoff[n];
@@ -1637,7 +1662,7 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
                 if (kMode < 2)
                     outcome = solve_column<G, kMode == 1, NMAX, !kLds>(a, L, s, col);
                 else if (kMode < 4)
-                    outcome = backward_column<G, kMode == 3>(a, L, s, col);
+                    outcome = backward_column<G, kMode == 3, NMAX>(a, L, s, col);
                 else
                     outcome = solve_column_qrup<G, !kLds>(a, L, s, col);
             }
@@ -1690,8 +1715,7 @@ using ColumnKernel = void (*)(BuildArgs, ScratchLayout, const int*, const int*, 
  * |J| fits the unrolled width. */
 static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
 {
-    const bool regs = (mode == 0 || mode == 1) && L.capI <= group &&
-                      L.capJ <= (group == 16 ? 8 : (group == 32 ? 16 : 32));
+    const bool regs = mode != 4 && L.capI <= group && L.capJ <= (group == 16 ? 8 : (group == 32 ? 16 : 32));
 #define MSPAI_PICK(MODE)                                                                            \
     if (group == 16) return &spai_column_kernel<16, true, MODE, 0>;                                 \
     if (group == 32) return &spai_column_kernel<32, true, MODE, 0>;                                 \
@@ -1705,8 +1729,14 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
         if (group == 16) return regs ? &spai_column_kernel<16, true, 1, 8> : &spai_column_kernel<16, true, 1, 0>;
         if (group == 32) return regs ? &spai_column_kernel<32, true, 1, 16> : &spai_column_kernel<32, true, 1, 0>;
         return regs ? &spai_column_kernel<64, true, 1, 32> : &spai_column_kernel<64, true, 1, 0>;
-    case 2: MSPAI_PICK(2);
-    case 3: MSPAI_PICK(3);
+    case 2:
+        if (group == 16) return regs ? &spai_column_kernel<16, true, 2, 8> : &spai_column_kernel<16, true, 2, 0>;
+        if (group == 32) return regs ? &spai_column_kernel<32, true, 2, 16> : &spai_column_kernel<32, true, 2, 0>;
+        return regs ? &spai_column_kernel<64, true, 2, 32> : &spai_column_kernel<64, true, 2, 0>;
+    case 3:
+        if (group == 16) return regs ? &spai_column_kernel<16, true, 3, 8> : &spai_column_kernel<16, true, 3, 0>;
+        if (group == 32) return regs ? &spai_column_kernel<32, true, 3, 16> : &spai_column_kernel<32, true, 3, 0>;
+        return regs ? &spai_column_kernel<64, true, 3, 32> : &spai_column_kernel<64, true, 3, 0>;
     default: MSPAI_PICK(4);
     }
 #undef MSPAI_PICK

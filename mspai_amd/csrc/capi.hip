@@ -572,6 +572,9 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
         L.tie_per_group = align_up(L.stack + tie_stack, 256);
     }
     L.per_wave = align_up(o, 256);
+    /* chunk size cap: cheap columns (small bins) in large chunks, expensive
+     * ones in small chunks */
+    L.max_rounds = capI <= 32 ? 16 : (capI <= 64 ? 8 : 2);
     return L;
 }
 
@@ -672,20 +675,6 @@ static int waves_per_cu(const Bin& b, int mode)
     return w;
 }
 
-/* The layout with its chunk size for `expected` columns.  Every resident
- * wave starts on one chunk by its block index; when the columns fit in one
- * chunk per wave (at most 16 columns per group), that static split is all
- * (no queue atomics), else chunks of 16 columns per group follow through the
- * queue atomic (all waves take the same address, so fewer, larger chunks beat
- * finer balance).  A bin that gets only overflow or fallback columns takes
- * one column per group. */
-static ScratchLayout with_rounds(ScratchLayout L, long long expected, int nwaves, int group)
-{
-    const long long groups = std::max(1LL, (long long)nwaves * (64 / group));
-    L.rounds = (int)std::max(1LL, std::min(16LL, (expected + groups - 1) / groups));
-    return L;
-}
-
 /* Launch routing and the bin chain over the columns whose patterns are in
  * a.p_ptr/p_ind.  Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb =
  * generic), mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills
@@ -701,6 +690,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     for (int b = 0; b < nb; ++b) {
         edges.capJ[b] = bins[b].L.capJ;
         edges.capSort[b] = bins[b].L.capSort;
+        edges.capI[b] = bins[b].L.capI;
     }
     int* counts = ctx->d_counter;
     int* heads = ctx->d_counter + 8;
@@ -735,7 +725,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
             a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
             int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
-            const ScratchLayout L = with_rounds(bins[b].L, b == 0 ? n : 0, ctx->num_cu * per_cu, grp);
+            const ScratchLayout& L = bins[b].L;
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
                           ctx->num_cu * per_cu, ctx->stream);
             HIPCHK(hipGetLastError());
@@ -753,9 +743,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         a.overflow = fq + (size_t)(b + 1) * n;
         a.n_overflow = counts + b + 1;
         int per_cu = waves_per_cu(bins[b], fp64_mode);
-        /* the first FP64 bin holds the routed columns unless the mixed chain
-         * ran first (then only its fallbacks arrive) */
-        const ScratchLayout L = with_rounds(bins[b].L, (b == 0 && !mixed) ? n : 0, ctx->num_cu * per_cu, grp);
+        const ScratchLayout& L = bins[b].L;
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());

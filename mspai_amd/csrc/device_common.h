@@ -48,7 +48,7 @@ struct ScratchLayout {
     int capSort;   // raw index lists before sorting (a power of two)
     int capPairs;  // candidates on the std::sort tie path
     int tie_in_main;        // pairs/stack in the main region (else in BuildArgs::tie_scratch)
-    int rounds;             // columns per group in one chunk of the work queue
+    int max_rounds;         // most columns per group in one chunk of the work queue
     size_t tie_per_group;   // bytes of a group's tie region when not in the main one
 };
 
@@ -56,6 +56,7 @@ struct ScratchLayout {
 struct BinEdges {
     int capJ[8];
     int capSort[8];
+    int capI[8];
 };
 
 /* Everything a column kernel needs; passed by value. */

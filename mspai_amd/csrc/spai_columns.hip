@@ -755,6 +755,203 @@ __device__ __forceinline__ int ls_solve_regs(const ScratchLayout& L, const WaveS
     return st;
 }
 
+/* Row-group layout for 64-lane groups: lane l = 16 c + q holds rows q,
+ * q + 16, q + 32, q + 48 (t = 0..3) of the columns j = c + 4 s (slot s =
+ * 0..7); the right-hand side is column 32 (c = 0, slot 8).  The dot product
+ * of a column adds its four rows in the lane first -- (p_t0 + p_t2) +
+ * (p_t1 + p_t3) are the xor-32 and xor-16 steps of the 64-lane order -- then
+ * the xor-8..1 steps run as DPP inside the 16-lane row, so every row group
+ * reduces its own column and no broadcast of the sums is needed.  The
+ * Householder vector of step i lives in row group i mod 4 and is copied to
+ * the other three with lane swaps. */
+
+/* value X of row group `src` (0..3) in every row group (same q) */
+__device__ __forceinline__ unsigned rowgroup_bcast(unsigned x, int src)
+{
+    auto a = __builtin_amdgcn_permlane16_swap(x, x, false, false);  // a[0]: rows 0,0,2,2; a[1]: 1,1,3,3
+    const unsigned z = (src & 1) ? a[1] : a[0];
+    auto b = __builtin_amdgcn_permlane32_swap(z, z, false, false);  // b[0]: z rows 0,1,0,1; b[1]: 2,3,2,3
+    return (src & 2) ? b[1] : b[0];
+}
+__device__ __forceinline__ double rowgroup_bcast(double x, int src)
+{
+    const long long v = __double_as_longlong(x);
+    const unsigned lo = rowgroup_bcast((unsigned)v, src), hi = rowgroup_bcast((unsigned)(v >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+/* value X of row group SRC (compile time) in row group 0 */
+template <int SRC>
+__device__ __forceinline__ unsigned rowgroup_to0(unsigned x)
+{
+    if (SRC == 0) return x;
+    if (SRC == 1) return __builtin_amdgcn_permlane16_swap(x, x, false, false)[1];
+    if (SRC == 2) return __builtin_amdgcn_permlane32_swap(x, x, false, false)[1];
+    const unsigned y = __builtin_amdgcn_permlane16_swap(x, x, false, false)[1];
+    return __builtin_amdgcn_permlane32_swap(y, y, false, false)[1];
+}
+template <int SRC>
+__device__ __forceinline__ double rowgroup_to0(double x)
+{
+    const long long v = __double_as_longlong(x);
+    const unsigned lo = rowgroup_to0<SRC>((unsigned)v), hi = rowgroup_to0<SRC>((unsigned)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+/* the four in-lane rows, then xor 8..1 inside the 16-lane row */
+__device__ __forceinline__ double rowgroup_sum(const double (&p)[4])
+{
+    using Gr = Group<64>;
+    double v = (p[0] + p[2]) + (p[1] + p[3]);
+    v = Gr::sum_step<8>(v);
+    v = Gr::sum_step<4>(v);
+    v = Gr::sum_step<2>(v);
+    v = Gr::sum_step<1>(v);
+    return v;
+}
+
+template <bool kROut>
+__device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const WaveScratch& s, int m, int n)
+{
+    constexpr int NS = 8;  // column slots; slot NS holds b
+    const int lane = lane_id();
+    const int c = lane >> 4, q = lane & 15;
+    const int ld = L.capI;
+    double a[NS + 1][4];
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int j = c + 4 * sl, r = q + 16 * t;
+            a[sl][t] = (r < m && j < n) ? s.aqr[j * ld + r] : 0.0;
+        }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int r = q + 16 * t;
+        a[NS][t] = (c == 0 && r < m) ? s.b[r] : 0.0;
+    }
+    const int k = m < n ? m : n;
+    bool zero_diag = false;
+#pragma unroll
+    for (int si = 0; si < NS; ++si) {
+        if (4 * si >= k) break;
+#pragma unroll 1
+        for (int ci = 0; ci < 4; ++ci) {
+            const int i = 4 * si + ci;
+            if (i >= k) break;
+            const int ti = i >> 4, qi = i & 15;
+            /* column i sits in row group ci, slot si */
+            double p[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r = q + 16 * t;
+                p[t] = r > i ? __fma_rn(a[si][t], a[si][t], 0.0) : 0.0;
+            }
+            const double xn2 = Group<64>::lane_value(rowgroup_sum(p), 16 * ci);
+            double aii_l = a[si][0];
+#pragma unroll
+            for (int t = 1; t < 4; ++t)
+                if (t == ti) aii_l = a[si][t];
+            const double alpha = Group<64>::lane_value(aii_l, 16 * ci + qi);
+            double rii = alpha;
+            if (i + 1 < m && xn2 != 0.0) {
+                const double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
+                const double beta = -copysign(nrm, alpha);
+                const double tau = (beta - alpha) / beta;
+                const double scal = 1.0 / (alpha - beta);
+                rii = beta;
+                /* v over this lane's rows, formed in row group ci, copied to all */
+                double v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int r = q + 16 * t;
+                    const double vl = r > i ? a[si][t] * scal : (r == i ? 1.0 : 0.0);
+                    v[t] = rowgroup_bcast(vl, ci);
+                    /* R_ii into column i's own lane; rows below keep v (dead) */
+                    if (c == ci && r == i) a[si][t] = beta;
+                    else if (c == ci && r > i) a[si][t] = vl;
+                }
+                const double mt = -tau;
+#pragma unroll
+                for (int sl = si; sl <= NS; ++sl) {
+                    if ((4 * sl + 3 > i && 4 * sl < n) || sl == NS) {
+                        const int j = c + 4 * sl;
+                        const bool live = sl == NS ? c == 0 : (j > i && j < n);
+                        double pp[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) pp[t] = q + 16 * t >= i ? __fma_rn(v[t], a[sl][t], 0.0) : 0.0;
+                        const double w = rowgroup_sum(pp);
+                        const double tw = mt * w;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            if (live && q + 16 * t >= i) a[sl][t] = __fma_rn(v[t], tw, a[sl][t]);
+                    }
+                }
+            }
+            zero_diag |= c == ci && q == qi && rii == 0.0;
+        }
+    }
+    if (kROut) {
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int j = c + 4 * sl, r = q + 16 * t;
+                if (j < n && r <= j) s.rq[j * L.capJ + r] = a[sl][t];
+            }
+        __syncthreads();
+        if (m < n) return COL_UNDERDET;
+        return __ballot(zero_diag) != 0ull ? COL_SINGULAR : 0;
+    }
+    int st = 0;
+    if (m < n) {
+        st = COL_UNDERDET;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a[NS][t] = 0.0;
+    } else if (__ballot(zero_diag) != 0ull) {
+        st = COL_SINGULAR;
+    } else {
+        /* dtrsm order: K over compile-time columns, x_K to all, column K of R
+         * moved from its row group to row group 0 where b lives */
+#pragma unroll
+        for (int K = 4 * NS - 1; K >= 0; --K) {
+            if (K < n) {
+                const int cK = K & 3, sK = K >> 2, tK = K >> 4, qK = K & 15;
+                double xk = Group<64>::lane_value(a[NS][tK], qK);
+                const double rkk = Group<64>::lane_value(a[sK][tK], 16 * cK + qK);
+                if (xk != 0.0) {
+                    xk = xk / rkk;
+                    double rcol[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (t <= tK) {
+                            switch (cK) {
+                            case 0: rcol[t] = rowgroup_to0<0>(a[sK][t]); break;
+                            case 1: rcol[t] = rowgroup_to0<1>(a[sK][t]); break;
+                            case 2: rcol[t] = rowgroup_to0<2>(a[sK][t]); break;
+                            default: rcol[t] = rowgroup_to0<3>(a[sK][t]); break;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (t <= tK) {
+                            const int r = q + 16 * t;
+                            if (c == 0 && r == K) a[NS][t] = xk;
+                            else if (c == 0 && r < K) a[NS][t] = __fma_rn(-xk, rcol[t], a[NS][t]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int r = q + 16 * t;
+        if (c == 0 && r < n) s.b[r] = a[NS][t];
+    }
+    __syncthreads();
+    return st;
+}
+
 /* Residual_Norm (src/Spai_Sub.imp:915-941): r = A(I,J) x - b in dgemv's column
  * order with beta = -1, then sqrt of the sequential sum of squares.  With
  * `dense`, from A(I,J) in the working set; else from the sparse columns (a
@@ -1061,7 +1258,12 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
             return kRedoFp64;
         }
     } else if (regs) {
-        if (ls_solve_regs<G, (NMAX > 0 ? NMAX : 4), true>(L, s, m, n) & (COL_SINGULAR | COL_UNDERDET)) return kDone;
+        int st2;
+        if constexpr (G == kWave && NMAX == 32)
+            st2 = ls_solve_rows64<true>(L, s, m, n);
+        else
+            st2 = ls_solve_regs<G, (NMAX > 0 ? NMAX : 4), true>(L, s, m, n);
+        if (st2 & (COL_SINGULAR | COL_UNDERDET)) return kDone;
         R = s.rq;
         ldr = L.capJ;
     } else {
@@ -1569,7 +1771,10 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         } else if (a.cache_slots > 0) {
             st |= ls_solve_cached<G>(a, L, s, m, n);
         } else if (NMAX > 0 && m <= G && n <= NMAX) {
-            st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n);
+            if constexpr (G == kWave && NMAX == 32)
+                st |= ls_solve_rows64<false>(L, s, m, n);
+            else
+                st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n);
             dense = true;
         } else {
             st |= ls_solve<G>(L, s, m, n);
@@ -1621,16 +1826,22 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
 {
     extern __shared__ __align__(16) char smem[];
     constexpr int NG = kWave / G;
-    /* columns taken per chunk: one per group, L.rounds rounds; a wave's first
-     * chunk is its block index, later chunks come from the queue atomic */
-    const int rounds = L.rounds > 0 ? L.rounds : 1;
-    const int chunk = NG * rounds;
     const int g = Group<G>::index();
     char* base = kLds ? smem + (size_t)g * L.per_wave
                       : a.scratch + ((size_t)blockIdx.x * NG + g) * L.per_wave;
     char* tie = L.tie_in_main ? base : a.tie_scratch + ((size_t)blockIdx.x * NG + g) * L.tie_per_group;
     WaveScratch s = wave_scratch(base, tie, L);
     const int nq = *nqueue;
+    /* columns per chunk: one per group, `rounds` rounds.  Every wave starts on
+     * the chunk of its block index; when the queue fits in one chunk per wave
+     * (at most L.max_rounds columns per group) that static split is all, else
+     * further chunks follow through the queue atomic.  All waves take the
+     * same address, so cheap columns want few, large chunks; expensive ones
+     * (large bins) want small chunks against the tail. */
+    const long long groups = (long long)gridDim.x * NG;
+    const long long cap = L.max_rounds > 0 ? L.max_rounds : 1;
+    const int rounds = (int)max(1LL, min(cap, ((long long)nq + groups - 1) / groups));
+    const int chunk = NG * rounds;
     const long long first = (long long)gridDim.x * chunk;  // chunks handed out by block index
     for (long long pass = 0;; ++pass) {
         long long q0 = 0;
@@ -1676,9 +1887,43 @@ __global__ __launch_bounds__(64) void spai_column_kernel(BuildArgs a, ScratchLay
 
 }  // namespace
 
-/* Route every column to the first bin whose |J| and raw-row capacity hold
- * its start pattern (the exact size of I is only known once the column runs;
- * a column that still does not fit moves on by the overflow path). */
+/* Number of distinct rows of the columns J (nj <= 8), counting stops at
+ * limit + 1: a merge of the sorted row lists on one thread. */
+__device__ __forceinline__ int distinct_rows_upto(const BuildArgs& a, int p0, int nj, int limit)
+{
+    int head[8], end[8], cur[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        head[t] = end[t] = 0;
+        cur[t] = 0x7fffffff;
+        if (t < nj) {
+            const int j = a.p_ind[p0 + t];
+            head[t] = a.A.colptr[j];
+            end[t] = a.A.colptr[j + 1];
+            if (head[t] < end[t]) cur[t] = a.A.rowind[head[t]];
+        }
+    }
+    int count = 0;
+    while (count <= limit) {
+        int mn = 0x7fffffff;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) mn = min(mn, cur[t]);
+        if (mn == 0x7fffffff) break;
+        ++count;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (cur[t] == mn) {
+                ++head[t];
+                cur[t] = head[t] < end[t] ? a.A.rowind[head[t]] : 0x7fffffff;
+            }
+    }
+    return count;
+}
+
+/* Route every column to the first bin whose |J|, raw row-list and |I|
+ * capacities hold its start pattern (|I| is counted exactly when the raw list
+ * exceeds the bin's |I| and |J0| <= 8; a column that still does not fit,
+ * e.g. after augmentation, moves on by the overflow path). */
 __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, BinEdges edges, int* queues,
                                                        int* counts, int* gqueue, int* gcount, int n)
 {
@@ -1691,11 +1936,21 @@ __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, B
             int j = a.p_ind[e];
             raw += a.A.colptr[j + 1] - a.A.colptr[j];
         }
-        for (int b = 0; b < nbins; ++b)
-            if (nj <= edges.capJ[b] && raw <= edges.capSort[b]) {
-                bin = b;
-                break;
+        /* distinct rows, counted on demand up to limit + 1 (a count above
+         * `limit` is only a lower bound) */
+        int rows = -1, limit = -1;
+        for (int b = 0; b < nbins; ++b) {
+            if (nj > edges.capJ[b] || raw > edges.capSort[b]) continue;
+            if (raw > edges.capI[b] && nj <= 8) {
+                if (rows < 0 || (rows > limit && edges.capI[b] > limit)) {
+                    limit = edges.capI[b];
+                    rows = distinct_rows_upto(a, p0, nj, limit);
+                }
+                if (rows > edges.capI[b]) continue;
             }
+            bin = b;
+            break;
+        }
     }
     for (int b = 0; b < nbins; ++b)
         wave_append(col < n && bin == b, col, queues + (size_t)b * n, counts + b);

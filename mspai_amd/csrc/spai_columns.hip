@@ -349,6 +349,27 @@ __device__ __forceinline__ void normal_solve(const RT* q, int ld, double* v, int
     }
 }
 
+/* normal_solve with v_i on group lane i (n <= G): the same operations in
+ * the same order, the pivots broadcast from their lane instead of going
+ * through LDS with a barrier per step. */
+template <int G, typename RT>
+__device__ __forceinline__ double normal_solve_lanes(const RT* q, int ld, double v, int n)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    for (int K = 0; K < n; ++K) {  // R^T y = v
+        const double yk = Gr::lane_value(v, K) / (double)q[K * ld + K];
+        if (lane == K) v = yk;
+        else if (lane > K && lane < n) v = __fma_rn(-yk, (double)q[lane * ld + K], v);
+    }
+    for (int K = n - 1; K >= 0; --K) {  // R d = y
+        const double dk = Gr::lane_value(v, K) / (double)q[K * ld + K];
+        if (lane == K) v = dk;
+        else if (lane < K) v = __fma_rn(-dk, (double)q[K * ld + lane], v);
+    }
+    return v;
+}
+
 /* out[i] = sum_j C(I_i, J_j) w_j over I in FP64, from the CSC columns or
  * (dense) from A(I,J) in the working set; with `base`, out = base - C w. */
 template <int G>
@@ -596,6 +617,28 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
     const float* q = s.aq32;
     if (NMAX > 0 && m <= G && n <= NMAX) {
         if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kFallback;
+        /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
+         * no barrier except after publishing x for the next residual */
+        const bool row = lane < m;
+        double x = lane < n ? s.b[lane] : 0.0;
+        for (int it = 0; it < kRefineSteps; ++it) {
+            double r = row ? s.b0[lane] : 0.0;  // r = b0 - A x (FP64)
+            for (int jj = 0; jj < n; ++jj)
+                if (row) r = __fma_rn(-s.b[jj], s.aqr[jj * ld + lane], r);
+            double g = 0.0;  // g = A^T r, g_j on lane j
+            for (int jj = 0; jj < n; ++jj) {
+                const double p = Gr::sum_fixed(row ? __fma_rn(s.aqr[jj * ld + lane], r, 0.0) : 0.0);
+                if (lane == jj) g = p;
+            }
+            g = normal_solve_lanes<G>(q, ld, g, n);  // R^T R d = g, R from the FP32 factor
+            x += g;
+            const double dmx = Gr::max(lane < n ? fabs(g) : 0.0);
+            const double xmx = Gr::max(lane < n ? fabs(x) : 0.0);
+            if (lane < n) s.b[lane] = x;
+            __syncthreads();
+            if (dmx <= 1e-10 * xmx) return 0;
+        }
+        return kFallback;
     } else {
         if (qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
         /* x0 in FP32, column-oriented back substitution */

@@ -187,6 +187,40 @@ int mspai_bin_counts(const mspai_ctx* ctx, int* nbins, int* routed, int* queued)
  * context's stream and returns when y is written. */
 int mspai_apply(mspai_ctx* ctx, int which, int transpose, const double* d_x, double* d_y, void* stream);
 
+
+/*
+ * Block SPAI (-bs, src/imspai.cpp:208-210, 746-760; Spai_Unrestrained_Block,
+ * src/Spai_Unrestrained_Block.imp:33-205).
+ *
+ * mspai_set_block_size converts the uploaded (square) matrix into a block
+ * matrix (Matrix<T>::Convert_Block_Matrix, src/Matrix.imp:1280-1516): bs > 1
+ * constant blocks of bs columns, the last holding the remainder
+ * (Find_Constant_Blocks); bs = 0 variable blocks from the runs of consecutive
+ * rows at each diagonal (Find_Diagonal_Blocks, upper limit 1000000); bs = 1
+ * returns to scalar SPAI.  In block mode the start pattern is over block
+ * columns (mspai_set_pattern with nblocks columns; NULL = block diagonal, the
+ * pattern the reference's -pp 1 gives the block matrix), mspai_build runs
+ * block SPAI in FP64 against the identity, and M comes out as the scalar
+ * matrix of Scalar_Matrix (src/Matrix.imp, exact zeros dropped): download,
+ * apply and reuse_pattern (the block pattern) work as in scalar mode;
+ * mspai_download_column_info fills the first nblocks entries (per block
+ * column).  Blocks hold at most 32 columns.  The adjoint, the autotuner, QR
+ * updates, the cache, mixed precision and targets are scalar-only.
+ */
+int mspai_set_block_size(mspai_ctx* ctx, int bs);
+
+/* Block structure of the current block matrix: the number of blocks, stored
+ * blocks, block sizes (nblocks), and the block pattern as CSC
+ * (colptr[nblocks+1], rowind[nnzb]).  Any output may be NULL.  Returns
+ * MSPAI_ERR_STATE outside block mode. */
+int mspai_block_info(const mspai_ctx* ctx, int* nblocks, long long* nnzb, int* block_sizes, int* colptr,
+                     int* rowind);
+
+/* Block partition of an n x n CSC matrix without a context (host only):
+ * Find_Constant_Blocks for bs > 0, Find_Diagonal_Blocks for bs = 0
+ * (src/Matrix.imp:1345-1514).  sizes needs room for n entries. */
+int mspai_block_sizes(int n, const int* colptr, const int* rowind, int bs, int* sizes, int* nblocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,8 @@ EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspa
             "mspai_build", "mspai_result_nnz", "mspai_download_csc",
             "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts",
             "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
-            "mspai_backward", "mspai_autotune", "mspai_apply"]
+            "mspai_backward", "mspai_autotune", "mspai_apply",
+            "mspai_set_block_size", "mspai_block_info", "mspai_block_sizes"]
 
 
 class Opts(C.Structure):
@@ -138,6 +139,25 @@ class Builder:
         re-solve between pattern recomputations)."""
         self._chk(self.lib.mspai_reuse_pattern(self.h), "mspai_reuse_pattern")
 
+    def set_block_size(self, bs: int):
+        """-bs: bs > 1 constant blocks, 0 variable blocks, 1 scalar SPAI.  In
+        block mode set_pattern takes a pattern over block columns (None =
+        block diagonal) and build() runs block SPAI; M stays scalar."""
+        self._chk(self.lib.mspai_set_block_size(self.h, int(bs)), "mspai_set_block_size")
+
+    def block_info(self):
+        """(block sizes, block pattern of C as an nb x nb CSC pattern)."""
+        nb = C.c_int()
+        nnzb = C.c_longlong()
+        self._chk(self.lib.mspai_block_info(self.h, C.byref(nb), C.byref(nnzb), None, None, None),
+                  "mspai_block_info")
+        sizes = np.zeros(nb.value, np.int32)
+        ptr = np.zeros(nb.value + 1, np.int32)
+        ind = np.zeros(max(1, nnzb.value), np.int32)
+        self._chk(self.lib.mspai_block_info(self.h, None, None, _p(sizes), _p(ptr), _p(ind)), "mspai_block_info")
+        P = sp.csc_matrix((np.ones(nnzb.value), ind[:nnzb.value], ptr), shape=(nb.value, nb.value))
+        return sizes, P
+
     def set_target(self, B=None):
         if B is None:
             self._chk(self.lib.mspai_set_target(self.h, None, None, None), "mspai_set_target")
@@ -230,6 +250,22 @@ class Builder:
         self._chk(self.lib.mspai_download_column_info(self.h, _p(res), _p(ns), _p(stt), _p(wk)),
                   "mspai_download_column_info")
         return {"res_norm": res, "nsolve": ns, "status": stt, "work": wk.reshape(n, NWORK)}
+
+
+def block_sizes(A, bs: int) -> np.ndarray:
+    """Block partition of the square CSC matrix A (host only, no device):
+    bs > 0 constant blocks, bs = 0 the variable blocks of the diagonal runs."""
+    lib = load()
+    A = sp.csc_matrix(A)
+    A.sort_indices()
+    n = A.shape[1]
+    sizes = np.zeros(n, np.int32)
+    nb = C.c_int()
+    ip, ix = _i32(A.indptr), _i32(A.indices)
+    rc = lib.mspai_block_sizes(C.c_int(n), _p(ip), _p(ix), C.c_int(bs), _p(sizes), C.byref(nb))
+    if rc != 0:
+        raise MspaiError(f"mspai_block_sizes failed ({rc})")
+    return sizes[:nb.value].copy()
 
 
 def spai(prob, device: int = 0, **opts):

@@ -20,7 +20,7 @@ OUT = os.path.join(HERE, "libmspai_gpu.so")
 CLI = os.path.join(HERE, "mspai_gpu")
 CLI_SOURCES = ["host/mspai_cli.cpp", "host/mmio.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["capi.hip", "spai_columns.hip", "apply.hip"]
+SOURCES = ["capi.hip", "spai_columns.hip", "apply.hip", "spai_block.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]

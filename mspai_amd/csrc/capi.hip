@@ -147,6 +147,22 @@ struct mspai_ctx {
     long long nnzB = 0;
     int last_routed[8] = {0};
     int last_queued[8] = {0};
+    /* block SPAI (-bs): 1 = scalar, > 1 constant blocks, 0 = variable blocks */
+    int block_bs = 1;
+    int nb = 0, maxbs = 1, bcmax = 0, brmax = 0;
+    long long nnzb = 0;
+    std::vector<int> h_bsize, h_bcolptr, h_browind;
+    int *d_bsize = nullptr, *d_bstart = nullptr, *d_bcolptr = nullptr, *d_browind = nullptr;
+    int *d_brp_ptr = nullptr, *d_brp_ind = nullptr;
+    long long *d_bvoff = nullptr, *d_sqoff = nullptr;
+    double *d_bval = nullptr, *d_sqinv = nullptr, *d_sqtmp = nullptr;
+    int *d_bm_len = nullptr, *d_bm_idx = nullptr;
+    double* d_bm_val = nullptr;
+    int bcapJ = 0;
+    long long bcapX = 0;
+    bool block_built = false;
+    char* d_bscratch = nullptr;
+    size_t bscratch_bytes = 0;
 };
 
 #define HIPCHK(x)                                                                              \
@@ -173,6 +189,37 @@ static int upload(mspai_ctx* ctx, T*& d, const T* h, size_t n)
     if (h) HIPCHK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     return MSPAI_OK;
 }
+
+/* Drop the block matrix and the block results (back to scalar SPAI). */
+static void block_free(mspai_ctx* ctx)
+{
+    dfree(ctx->d_bsize);
+    dfree(ctx->d_bstart);
+    dfree(ctx->d_bcolptr);
+    dfree(ctx->d_browind);
+    dfree(ctx->d_brp_ptr);
+    dfree(ctx->d_brp_ind);
+    dfree(ctx->d_bvoff);
+    dfree(ctx->d_sqoff);
+    dfree(ctx->d_bval);
+    dfree(ctx->d_sqinv);
+    dfree(ctx->d_sqtmp);
+    dfree(ctx->d_bm_len);
+    dfree(ctx->d_bm_idx);
+    dfree(ctx->d_bm_val);
+    dfree(ctx->d_bscratch);
+    ctx->bscratch_bytes = 0;
+    ctx->bcapJ = 0;
+    ctx->bcapX = 0;
+    ctx->block_built = false;
+    ctx->block_bs = 1;
+    ctx->nb = 0;
+    ctx->h_bsize.clear();
+    ctx->h_bcolptr.clear();
+    ctx->h_browind.clear();
+}
+
+static int block_convert(mspai_ctx* ctx);
 
 extern "C" void mspai_default_opts(mspai_opts* o)
 {
@@ -265,6 +312,7 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_mcsr_ind);
     dfree(ctx->d_mcsr_val);
     dfree(ctx->d_csr_cursor);
+    block_free(ctx);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -330,6 +378,11 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     ctx->a_csr_ok = ctx->m_csr_ok = false;
     ctx->pattern_nnz = -1;
     ctx->tuned_table = -1;
+    if (ctx->block_bs != 1) {
+        /* a new matrix leaves block mode; its block pattern no longer applies */
+        block_free(ctx);
+        ctx->pattern_set = false;
+    }
     if (!ctx->pattern_set) {
         dfree(ctx->d_p_ptr);
         dfree(ctx->d_p_ind);
@@ -345,6 +398,7 @@ extern "C" int mspai_update_values(mspai_ctx* ctx, const double* val)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
     ctx->a_csr_ok = ctx->m_csr_ok = false;
+    if (ctx->block_bs != 1) return block_convert(ctx);  // Convert_Block_Matrix_Update
     return MSPAI_OK;
 }
 
@@ -353,7 +407,7 @@ extern "C" int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* r
     if (!ctx) return MSPAI_ERR_ARG;
     if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "set_pattern: upload the matrix first");
     HIPCHK(hipSetDevice(ctx->device));
-    int n = ctx->ncols;
+    const int n = ctx->block_bs != 1 ? ctx->nb : ctx->ncols;  // block mode: a pattern over block columns
     std::vector<int> dp, di;
     if (!colptr || !rowind) {
         dp.resize(n + 1);
@@ -388,6 +442,20 @@ extern "C" int mspai_reuse_pattern(mspai_ctx* ctx)
     if (!ctx) return MSPAI_ERR_ARG;
     if (ctx->pattern_nnz < 0) return fail(ctx, MSPAI_ERR_STATE, "reuse_pattern: build M first");
     HIPCHK(hipSetDevice(ctx->device));
+    if (ctx->block_bs != 1) {
+        /* P_Memory of a block build is the block pattern of M */
+        if (!ctx->block_built) return fail(ctx, MSPAI_ERR_STATE, "reuse_pattern: build M first");
+        const int nb = ctx->nb, cj = ctx->bcapJ;
+        std::vector<int> len(nb), idx((size_t)nb * cj), bp(nb + 1, 0), bi;
+        HIPCHK(hipMemcpy(len.data(), ctx->d_bm_len, nb * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(idx.data(), ctx->d_bm_idx, (size_t)nb * cj * sizeof(int), hipMemcpyDeviceToHost));
+        for (int c = 0; c < nb; ++c) {
+            bp[c + 1] = bp[c] + len[c];
+            bi.insert(bi.end(), idx.begin() + (size_t)c * cj, idx.begin() + (size_t)c * cj + len[c]);
+        }
+        if (bi.empty()) bi.push_back(0);
+        return mspai_set_pattern(ctx, bp.data(), bi.data());
+    }
     std::vector<int> ptr(ctx->ncols + 1), ind(std::max<long long>(ctx->pattern_nnz, 1));
     HIPCHK(hipMemcpy(ptr.data(), ctx->d_Mptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
     if (ctx->pattern_nnz > 0)
@@ -839,6 +907,8 @@ static BuildArgs base_args(mspai_ctx* ctx)
     return a;
 }
 
+static int build_block(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats);
+
 extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats)
 {
     if (!ctx || !opts) return MSPAI_ERR_ARG;
@@ -846,6 +916,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     int rc;
     if ((rc = check_opts(ctx, opts))) return rc;
     HIPCHK(hipSetDevice(ctx->device));
+    if (ctx->block_bs != 1) return build_block(ctx, opts, stats);
     if (!ctx->d_p_ptr) {
         if ((rc = mspai_set_pattern(ctx, nullptr, nullptr))) return rc;
     }
@@ -946,6 +1017,7 @@ extern "C" int mspai_update_values_device(mspai_ctx* ctx, const double* d_val)
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
     ctx->a_csr_ok = ctx->m_csr_ok = false;
+    if (ctx->block_bs != 1) return block_convert(ctx);
     return MSPAI_OK;
 }
 
@@ -974,6 +1046,7 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
                               double* d_gB, mspai_stats* stats)
 {
     if (!ctx || !opts || !d_gM || !d_gC) return MSPAI_ERR_ARG;
+    if (ctx->block_bs != 1) return fail(ctx, MSPAI_ERR_UNSUPPORTED, "backward: not available for block SPAI");
     if (ctx->nnzM < 0) return fail(ctx, MSPAI_ERR_STATE, "backward: build M first (values unchanged since)");
     if (d_gB && !ctx->d_b_ptr) return fail(ctx, MSPAI_ERR_ARG, "backward: gB given but no target set");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1024,6 +1097,7 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
 extern "C" int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeats, mspai_tune_report* report)
 {
     if (!ctx || !opts) return MSPAI_ERR_ARG;
+    if (ctx->block_bs != 1) return fail(ctx, MSPAI_ERR_UNSUPPORTED, "autotune: not available for block SPAI");
     if (repeats < 1) repeats = 1;
     mspai_opts o = *opts;
     o.binning = 2;
@@ -1179,3 +1253,389 @@ extern "C" int mspai_apply(mspai_ctx* ctx, int which, int transpose, const doubl
     return MSPAI_OK;
 }
 
+
+/* ------------------------------------------------------------------ block SPAI */
+
+namespace {
+
+constexpr int kMaxBlock = 32;                        // largest block size of the device path
+constexpr size_t kBlockScratchBudget = 8ull << 30;   // HBM for the block kernel's working sets
+
+/* Initial_Run_Length (src/Matrix.imp:1430-1462): rows i, i+1, ... stored
+ * consecutively in column i from its diagonal entry on (1 without one). */
+int initial_run_length(const int* cp, const int* ri, int i)
+{
+    int k = cp[i];
+    while (k < cp[i + 1] && ri[k] != i) ++k;
+    if (k == cp[i + 1]) return 1;
+    int bs = 1, ii = i + 1;
+    for (++k; k < cp[i + 1]; ++k) {
+        if (ri[k] != ii) return bs;
+        ++bs;
+        ++ii;
+    }
+    return bs;
+}
+
+/* Check_Next_Run (src/Matrix.imp:1464-1514): the run of column i_next from
+ * row i must reach i_next's diagonal; then min(bs, run), else 0. */
+int check_next_run(const int* cp, const int* ri, int i, int i_next, int bs)
+{
+    int k = cp[i_next];
+    while (k < cp[i_next + 1] && ri[k] != i) ++k;
+    if (k == cp[i_next + 1]) return 0;
+    int nbs = 1, ii = i + 1;
+    for (++k; k < cp[i_next + 1]; ++k) {
+        if (ri[k] != ii) break;
+        ++nbs;
+        ++ii;
+    }
+    if (nbs > i_next - i) return nbs > bs ? bs : nbs;
+    return 0;
+}
+
+/* Find_Diagonal_Block (src/Matrix.imp:1404-1428): last column of the block
+ * that starts at column i. */
+int find_diagonal_block(int n, const int* cp, const int* ri, int i, int limit)
+{
+    int max_bs = initial_run_length(cp, ri, i);
+    int bs = max_bs;
+    if (max_bs == 1) return i;
+    if (limit && max_bs > limit) bs = max_bs = limit;
+    for (int k = 0; k < max_bs; ++k) {
+        if (k > 0 && i + k == n) return i + k - 1;
+        bs = check_next_run(cp, ri, i, i + k, bs);
+        if (!bs) return i + k - 1;
+    }
+    return i + max_bs - 1;
+}
+
+}  // namespace
+
+extern "C" int mspai_block_sizes(int n, const int* colptr, const int* rowind, int bs, int* sizes, int* nblocks)
+{
+    if (n <= 0 || !sizes || !nblocks || bs < 0) return MSPAI_ERR_ARG;
+    int nbk = 0;
+    if (bs > 0) {
+        /* Find_Constant_Blocks (src/Matrix.imp:1345-1363) */
+        for (int i = 0; i < n / bs; ++i) sizes[nbk++] = bs;
+        if (n % bs) sizes[nbk++] = n % bs;
+    } else {
+        /* Find_Diagonal_Blocks (src/Matrix.imp:1381-1402), upper limit of src/imspai.cpp:754-755 */
+        if (!colptr || !rowind) return MSPAI_ERR_ARG;
+        for (int i = 0; i < n;) {
+            int iend = find_diagonal_block(n, colptr, rowind, i, 1000000);
+            sizes[nbk++] = iend - i + 1;
+            i = iend + 1;
+        }
+    }
+    *nblocks = nbk;
+    return MSPAI_OK;
+}
+
+/* Convert_To_Block_Matrix (src/Matrix.imp:1516-1745) of the uploaded matrix
+ * with block sizes ctx->h_bsize: block CSC with zero-filled column-major
+ * blocks, the block row pattern, and the offsets of (A_j^T A_j)^-1. */
+static int block_convert(mspai_ctx* ctx)
+{
+    const int n = ctx->ncols, nb = (int)ctx->h_bsize.size();
+    const long long nnz = ctx->nnz;
+    std::vector<int> cp(n + 1), ri(std::max<long long>(nnz, 1));
+    std::vector<double> v(std::max<long long>(nnz, 1));
+    HIPCHK(hipMemcpy(cp.data(), ctx->d_colptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ri.data(), ctx->d_rowind, nnz * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(v.data(), ctx->d_val, nnz * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int> start(nb + 1, 0), bmap(n);
+    int maxbs = 1;
+    for (int b = 0; b < nb; ++b) {
+        start[b + 1] = start[b] + ctx->h_bsize[b];
+        maxbs = std::max(maxbs, ctx->h_bsize[b]);
+        for (int i = start[b]; i < start[b + 1]; ++i) bmap[i] = b;
+    }
+    if (maxbs > kMaxBlock)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED,
+                    "set_block_size: block of " + std::to_string(maxbs) + " columns; the device path takes at most " +
+                        std::to_string(kMaxBlock));
+    std::vector<int> bcp(nb + 1, 0), bri, pos(nb, -1);
+    std::vector<long long> voff(1, 0);
+    std::vector<double> bv;
+    int bcmax = 0;
+    for (int jb = 0; jb < nb; ++jb) {
+        const int w = ctx->h_bsize[jb];
+        std::vector<int> rows;
+        for (int j = start[jb]; j < start[jb + 1]; ++j)
+            for (int e = cp[j]; e < cp[j + 1]; ++e) {
+                int ib = bmap[ri[e]];
+                if (pos[ib] < 0) {
+                    pos[ib] = 0;
+                    rows.push_back(ib);
+                }
+            }
+        std::sort(rows.begin(), rows.end());
+        const size_t base = bv.size();
+        for (size_t t = 0; t < rows.size(); ++t) {
+            const int ib = rows[t], h = ctx->h_bsize[ib];
+            pos[ib] = (int)(voff.back() - (long long)base);
+            bv.resize(bv.size() + (size_t)h * w, 0.0);
+            voff.push_back(voff.back() + (long long)h * w);
+            bri.push_back(ib);
+        }
+        for (int j = start[jb]; j < start[jb + 1]; ++j)
+            for (int e = cp[j]; e < cp[j + 1]; ++e) {
+                const int ib = bmap[ri[e]], h = ctx->h_bsize[ib];
+                bv[base + pos[ib] + (size_t)(j - start[jb]) * h + (ri[e] - start[ib])] = v[e];
+            }
+        for (int ib : rows) pos[ib] = -1;
+        bcp[jb + 1] = (int)bri.size();
+        bcmax = std::max(bcmax, (int)rows.size());
+    }
+    /* block row structure: the transpose of the block pattern */
+    const long long nnzb = bcp[nb];
+    std::vector<int> rp(nb + 1, 0), rind(std::max<long long>(nnzb, 1));
+    for (long long e = 0; e < nnzb; ++e) rp[bri[e] + 1]++;
+    int brmax = 0;
+    for (int b = 0; b < nb; ++b) {
+        brmax = std::max(brmax, rp[b + 1]);
+        rp[b + 1] += rp[b];
+    }
+    {
+        std::vector<int> fill(rp.begin(), rp.end() - 1);
+        for (int jb = 0; jb < nb; ++jb)
+            for (int e = bcp[jb]; e < bcp[jb + 1]; ++e) rind[fill[bri[e]]++] = jb;
+    }
+    std::vector<long long> sqoff(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) sqoff[b + 1] = sqoff[b] + (long long)ctx->h_bsize[b] * ctx->h_bsize[b];
+    ctx->h_bcolptr = bcp;
+    ctx->h_browind = bri;
+    if (bri.empty()) bri.push_back(0);  // uploads take at least one element
+    if (bv.empty()) bv.push_back(0.0);
+    int rc;
+    if ((rc = upload(ctx, ctx->d_bsize, ctx->h_bsize.data(), nb))) return rc;
+    if ((rc = upload(ctx, ctx->d_bstart, start.data(), nb + 1))) return rc;
+    if ((rc = upload(ctx, ctx->d_bcolptr, bcp.data(), nb + 1))) return rc;
+    if ((rc = upload(ctx, ctx->d_browind, bri.data(), bri.size()))) return rc;
+    if ((rc = upload(ctx, ctx->d_bvoff, voff.data(), voff.size()))) return rc;
+    if ((rc = upload(ctx, ctx->d_bval, bv.data(), bv.size()))) return rc;
+    if ((rc = upload(ctx, ctx->d_brp_ptr, rp.data(), nb + 1))) return rc;
+    if ((rc = upload(ctx, ctx->d_brp_ind, rind.data(), nnzb))) return rc;
+    if ((rc = upload(ctx, ctx->d_sqoff, sqoff.data(), nb + 1))) return rc;
+    if ((rc = upload<double>(ctx, ctx->d_sqinv, nullptr, sqoff[nb]))) return rc;
+    if ((rc = upload<double>(ctx, ctx->d_sqtmp, nullptr, sqoff[nb]))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nb = nb;
+    ctx->maxbs = maxbs;
+    ctx->bcmax = bcmax;
+    ctx->brmax = brmax;
+    ctx->nnzb = nnzb;
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_set_block_size(mspai_ctx* ctx, int bs)
+{
+    if (!ctx) return MSPAI_ERR_ARG;
+    if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "set_block_size: upload the matrix first");
+    if (bs < 0) return fail(ctx, MSPAI_ERR_ARG, "Illegal block size (bs < 0)");
+    HIPCHK(hipSetDevice(ctx->device));
+    const bool was_block = ctx->block_bs != 1;
+    if (bs == 1) {
+        if (was_block) {
+            block_free(ctx);
+            dfree(ctx->d_p_ptr);
+            dfree(ctx->d_p_ind);
+            ctx->pattern_set = false;
+        }
+        ctx->nnzM = -1;
+        return MSPAI_OK;
+    }
+    if (ctx->nrows != ctx->ncols)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED, "block SPAI needs a square matrix (no probing rows)");
+    const int n = ctx->ncols;
+    std::vector<int> sizes(n);
+    int nbk = 0;
+    std::vector<int> cp, ri;
+    if (bs == 0) {
+        cp.resize(n + 1);
+        ri.resize(std::max<long long>(ctx->nnz, 1));
+        HIPCHK(hipMemcpy(cp.data(), ctx->d_colptr, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ri.data(), ctx->d_rowind, ctx->nnz * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    int rc = mspai_block_sizes(n, cp.empty() ? nullptr : cp.data(), ri.empty() ? nullptr : ri.data(), bs,
+                               sizes.data(), &nbk);
+    if (rc) return fail(ctx, rc, "set_block_size: block detection failed");
+    block_free(ctx);
+    sizes.resize(nbk);
+    ctx->h_bsize = sizes;
+    if ((rc = block_convert(ctx))) {
+        block_free(ctx);
+        return rc;
+    }
+    ctx->block_bs = bs;
+    /* a start pattern set before now is over scalar columns: back to the
+     * block diagonal unless the caller sets a block pattern */
+    dfree(ctx->d_p_ptr);
+    dfree(ctx->d_p_ind);
+    ctx->pattern_set = false;
+    ctx->nnzM = -1;
+    ctx->pattern_nnz = -1;
+    ctx->m_csr_ok = false;
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_block_info(const mspai_ctx* ctx, int* nblocks, long long* nnzb, int* block_sizes, int* colptr,
+                                int* rowind)
+{
+    if (!ctx) return MSPAI_ERR_ARG;
+    if (ctx->block_bs == 1) return MSPAI_ERR_STATE;
+    if (nblocks) *nblocks = ctx->nb;
+    if (nnzb) *nnzb = ctx->nnzb;
+    if (block_sizes) std::copy(ctx->h_bsize.begin(), ctx->h_bsize.end(), block_sizes);
+    if (colptr) std::copy(ctx->h_bcolptr.begin(), ctx->h_bcolptr.end(), colptr);
+    if (rowind) std::copy(ctx->h_browind.begin(), ctx->h_browind.end(), rowind);
+    return MSPAI_OK;
+}
+
+/* Spai<T>::SPAI_Algorithm with Spai_Unrestrained_Block (Switch_Algorithm,
+ * src/Switch_Algorithm.imp:65-69) and Scalar_Matrix at the end
+ * (src/imspai.cpp:858-870). */
+static int build_block(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats)
+{
+    if (opts->precision != 0) return fail(ctx, MSPAI_ERR_UNSUPPORTED, "block SPAI runs in FP64 only");
+    if (opts->qr != 0 || opts->cache != 0)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED, "block SPAI takes neither QR updates (-qr) nor the cache (-cs)");
+    if (ctx->d_b_ptr)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED, "block SPAI builds against the identity target only");
+    int rc;
+    if (!ctx->d_p_ptr) {
+        if ((rc = mspai_set_pattern(ctx, nullptr, nullptr))) return rc;
+    }
+    const int n = ctx->ncols, nb = ctx->nb, mb = ctx->maxbs;
+    const int ns = opts->max_impr_steps;
+    const int capJ = std::min<long long>(nb, (long long)ctx->plen_max + (long long)opts->maxnew * ns);
+    if (capJ <= 0) return fail(ctx, MSPAI_ERR_ARG, "build: empty start pattern and no augmentation");
+    const long long capX = (long long)capJ * mb * mb;
+    BlockLayout L{};
+    L.maxbs = mb;
+    L.capJ = capJ;
+    L.capJs = (int)std::min<long long>(n, (long long)capJ * mb);
+    L.capI = (int)std::min<long long>(nb, (long long)capJ * std::max(1, ctx->bcmax));
+    L.capIs = (int)std::min<long long>(n, (long long)L.capI * mb);
+    const long long rawI = (long long)capJ * std::max(1, ctx->bcmax);
+    const long long rawU = ns > 0 ? (long long)(L.capI + 1) * std::max(1, ctx->brmax) : 1;
+    const long long rawmax = std::max<long long>(std::max(rawI, rawU), 64);
+    if (rawmax > (1 << 26)) return fail(ctx, MSPAI_ERR_CAPACITY, "block build: index lists too long");
+    L.capSort = (int)next_pow2((size_t)rawmax);
+    L.capU = L.capSort;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    L.J = take(capJ * sizeof(int));
+    L.I = take(L.capSort * sizeof(int));
+    L.sbuf = take(L.capSort * sizeof(int));
+    L.jt = take(L.capSort * sizeof(int));
+    L.fr = take((L.capSort + 1) * sizeof(int));
+    L.fc = take((capJ + 1) * sizeof(int));
+    L.sel = take(capJ * sizeof(int));
+    L.stack = take(3 * introsort_detail::kStackFrames * sizeof(int));
+    L.pairs = take(L.capSort * sizeof(RhoPair));
+    L.rho = take(L.capSort * sizeof(double));
+    L.ahat = take((size_t)L.capIs * L.capJs * sizeof(double));
+    L.aqr = take((size_t)L.capIs * L.capJs * sizeof(double));
+    L.b = take((size_t)L.capIs * mb * sizeof(double));
+    L.b0 = take((size_t)L.capIs * mb * sizeof(double));
+    L.r = take((size_t)L.capIs * mb * sizeof(double));
+    L.rb = take((size_t)L.capIs * mb * sizeof(double));
+    L.x = take((size_t)L.capJs * mb * sizeof(double));
+    L.nbuf = take((size_t)kWave * 2 * mb * mb * sizeof(double));
+    L.per_wave = off;
+    long long nwaves = std::min<long long>(nb, (long long)ctx->num_cu * 8);
+    nwaves = std::min<long long>(nwaves, (long long)(kBlockScratchBudget / L.per_wave));
+    if (nwaves < 1) return fail(ctx, MSPAI_ERR_CAPACITY, "block build: working set of one column exceeds the budget");
+    const size_t sbytes = (size_t)nwaves * L.per_wave;
+    if (sbytes > ctx->bscratch_bytes) {
+        dfree(ctx->d_bscratch);
+        ctx->bscratch_bytes = 0;
+        HIPCHK(hipMalloc((void**)&ctx->d_bscratch, sbytes));
+        ctx->bscratch_bytes = sbytes;
+    }
+    if ((rc = ensure_outputs(ctx, (int)std::min<long long>(n, (long long)capJ * mb)))) return rc;
+    if (ctx->bcapJ != capJ || ctx->bcapX != capX || !ctx->d_bm_len) {
+        dfree(ctx->d_bm_len);
+        dfree(ctx->d_bm_idx);
+        dfree(ctx->d_bm_val);
+        HIPCHK(hipMalloc((void**)&ctx->d_bm_len, (size_t)nb * sizeof(int)));
+        HIPCHK(hipMalloc((void**)&ctx->d_bm_idx, (size_t)nb * capJ * sizeof(int)));
+        HIPCHK(hipMalloc((void**)&ctx->d_bm_val, (size_t)nb * capX * sizeof(double)));
+        ctx->bcapJ = capJ;
+        ctx->bcapX = capX;
+    }
+    BlockArgs a{};
+    a.nb = nb;
+    a.bsize = ctx->d_bsize;
+    a.bstart = ctx->d_bstart;
+    a.colptr = ctx->d_bcolptr;
+    a.rowind = ctx->d_browind;
+    a.voff = ctx->d_bvoff;
+    a.val = ctx->d_bval;
+    a.rp_ptr = ctx->d_brp_ptr;
+    a.rp_ind = ctx->d_brp_ind;
+    a.p_ptr = ctx->d_p_ptr;
+    a.p_ind = ctx->d_p_ind;
+    a.sqinv = ctx->d_sqinv;
+    a.sqoff = ctx->d_sqoff;
+    a.epsilon = opts->epsilon;
+    a.maxnew = opts->maxnew;
+    a.ns = ns;
+    a.use_mean = opts->use_mean;
+    a.capJ = capJ;
+    a.capX = capX;
+    a.m_len = ctx->d_bm_len;
+    a.m_idx = ctx->d_bm_idx;
+    a.m_val = ctx->d_bm_val;
+    a.res = ctx->d_res;
+    a.nsolve = ctx->d_nsolve;
+    a.status = ctx->d_status;
+    a.head = ctx->d_counter + 56;
+    a.error = ctx->d_counter + 57;
+    a.scratch = ctx->d_bscratch;
+
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_counter + 56, 0, 2 * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_work, 0, (size_t)n * kNumWork * sizeof(double), ctx->stream));
+    if (ns > 0) launch_block_sq_inverse(a, ctx->d_sqtmp, ctx->stream);
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    launch_block_columns(a, L, (int)nwaves, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    /* Scalar_Matrix: counts per scalar column, scan, fill */
+    launch_block_scalar_count(a, ctx->d_m_len, ctx->stream);
+    size_t cub_bytes = ctx->cub_bytes;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(ctx->d_cub, cub_bytes, ctx->d_m_len, ctx->d_Mptr + 1, n, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_Mptr, 0, sizeof(int), ctx->stream));
+    launch_block_scalar_fill(a, ctx->d_Mptr, ctx->d_Mind, ctx->d_Mval, ctx->stream);
+    HIPCHK(hipGetLastError());
+    launch_copy_ints(ctx->d_Mptr + n, ctx->d_counter + 63, 1, ctx->stream);
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    if ((rc = fetch_counters(ctx))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_counter[57])
+        return fail(ctx, MSPAI_ERR_CAPACITY, "block build: a block column outgrew its working set");
+    const int last = ctx->h_counter[63];
+    ctx->nnzM = last;
+    ctx->pattern_nnz = last;
+    ctx->m_csr_ok = false;
+    ctx->block_built = true;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float t_all = 0, t_solve = 0;
+        HIPCHK(hipEventElapsedTime(&t_all, ctx->ev[0], ctx->ev[3]));
+        HIPCHK(hipEventElapsedTime(&t_solve, ctx->ev[1], ctx->ev[2]));
+        stats->build_ms = t_all;
+        stats->solve_ms = t_solve;
+        stats->nnz = last;
+        stats->ncols = nb;
+    }
+    return MSPAI_OK;
+}

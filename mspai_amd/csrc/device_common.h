@@ -103,6 +103,51 @@ struct BuildArgs {
     char* tie_scratch;      // per-group HBM tie-path buffers of the LDS bins
 };
 
+/* Block SPAI (-bs != 1): the block matrix, its patterns and the outputs.
+ * Block e of block column j is stored column-major (bsize[rowind[e]] x
+ * bsize[j]) at val[voff[e]]. */
+struct BlockArgs {
+    int nb;                   // block columns (= block rows)
+    const int* bsize;         // nb
+    const int* bstart;        // nb + 1 scalar offsets
+    const int* colptr;        // block CSC
+    const int* rowind;
+    const long long* voff;
+    const double* val;
+    const int* rp_ptr;        // block row pattern (CSR)
+    const int* rp_ind;
+    const int* p_ptr;         // start pattern over block columns (CSC)
+    const int* p_ind;
+    const double* sqinv;      // (A_j^T A_j)^-1, column-major, at sqoff[j]
+    const long long* sqoff;
+    double epsilon;
+    int maxnew, ns, use_mean;
+    int capJ;                 // blocks per output column
+    long long capX;           // values per output column
+    int* m_len;               // blocks of M's block column
+    int* m_idx;               // their block rows, nb x capJ
+    double* m_val;            // their values, nb x capX (Get_Solution_Vals_Block order)
+    double* res;
+    int* nsolve;
+    int* status;
+    int* head;                // work counter of the persistent kernel
+    int* error;               // set when a column outgrew the working set
+    char* scratch;            // per-wave working sets
+};
+
+/* Per-wave working set of the block kernel (byte offsets) and its sizes. */
+struct BlockLayout {
+    size_t J, I, sbuf, jt, fr, fc, sel, stack, pairs, rho, ahat, aqr, b, b0, r, rb, x, nbuf;
+    size_t per_wave;
+    int capJ;     // blocks in J
+    int capI;     // blocks in I
+    int capIs;    // scalar rows of A(I,J) (its leading dimension)
+    int capJs;    // scalar columns of A(I,J)
+    int capU;     // candidates
+    int capSort;  // raw index lists before sorting (a power of two)
+    int maxbs;    // largest block size
+};
+
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
 /*
@@ -551,6 +596,58 @@ __device__ __forceinline__ int group_sort_unique(int* buf, int count, int* out)
 }
 
 __device__ inline int wave_sort_unique(int* buf, int count, int* out) { return group_sort_unique<64>(buf, count, out); }
+
+/* c <- (I - tau v v^T) c over rows i..m-1, v_i = 1 (dlarf('L')). */
+template <int G>
+__device__ __forceinline__ void apply_reflector(const double* v, double* c, int i, int m, double tau)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    double p = 0.0;
+    for (int r = lane; r < m; r += G) {
+        if (r < i) continue;
+        double vr = (r == i) ? 1.0 : v[r];
+        p = __fma_rn(vr, c[r], p);
+    }
+    double w = Gr::sum_fixed(p);
+    double t = (-tau) * w;
+    for (int r = lane; r < m; r += G) {
+        if (r < i) continue;
+        double vr = (r == i) ? 1.0 : v[r];
+        c[r] = __fma_rn(vr, t, c[r]);
+    }
+    __syncthreads();
+}
+
+/* Householder vector of storage column c over storage rows [c, rows)
+ * (dlarfg in the fixed order).  Returns tau. */
+template <int G>
+__device__ __forceinline__ double make_reflector(double* ai, int c, int rows)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    double tau = 0.0;
+    if (c + 1 < rows) {
+        double alpha = ai[c];
+        double p = 0.0;
+        for (int r = lane; r < rows; r += G)
+            if (r > c) p = __fma_rn(ai[r], ai[r], p);
+        double xn2 = Gr::sum_fixed(p);
+        if (xn2 != 0.0) {
+            double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
+            double beta = -copysign(nrm, alpha);
+            tau = (beta - alpha) / beta;
+            double scal = 1.0 / (alpha - beta);
+            __syncthreads();
+            for (int r = lane; r < rows; r += G)
+                if (r > c) ai[r] = ai[r] * scal;
+            if (lane == 0) ai[c] = beta;
+            __syncthreads();
+        }
+    }
+    return tau;
+}
+
 
 }  // namespace mspai
 

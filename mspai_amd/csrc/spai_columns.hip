@@ -245,28 +245,6 @@ __device__ __forceinline__ int make_rhs(const BuildArgs& a, const WaveScratch& s
     return hit ? 0 : COL_K_NOT_IN_I;
 }
 
-/* c <- (I - tau v v^T) c over rows i..m-1, v_i = 1 (dlarf('L')). */
-template <int G>
-__device__ __forceinline__ void apply_reflector(const double* v, double* c, int i, int m, double tau)
-{
-    using Gr = Group<G>;
-    const int lane = Gr::lane();
-    double p = 0.0;
-    for (int r = lane; r < m; r += G) {
-        if (r < i) continue;
-        double vr = (r == i) ? 1.0 : v[r];
-        p = __fma_rn(vr, c[r], p);
-    }
-    double w = Gr::sum_fixed(p);
-    double t = (-tau) * w;
-    for (int r = lane; r < m; r += G) {
-        if (r < i) continue;
-        double vr = (r == i) ? 1.0 : v[r];
-        c[r] = __fma_rn(vr, t, c[r]);
-    }
-    __syncthreads();
-}
-
 /* Approximative_Solution (src/Spai_Unrestrained.imp:135-195): QR, Q^T b,
  * dtrtrs.  On exit b[0..n) holds the solution.  Returns status bits. */
 template <int G>
@@ -1370,35 +1348,6 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
 
 /* One column (Spai_Unrestrained<T>::SPAI_Column).  kMoveOn: it does not fit
  * this launch's layout; kRedoFp64: the mixed path declined a decision. */
-/* Householder vector of storage column c over storage rows [c, rows)
- * (dlarfg in the fixed order).  Returns tau. */
-template <int G>
-__device__ __forceinline__ double make_reflector(double* ai, int c, int rows)
-{
-    using Gr = Group<G>;
-    const int lane = Gr::lane();
-    double tau = 0.0;
-    if (c + 1 < rows) {
-        double alpha = ai[c];
-        double p = 0.0;
-        for (int r = lane; r < rows; r += G)
-            if (r > c) p = __fma_rn(ai[r], ai[r], p);
-        double xn2 = Gr::sum_fixed(p);
-        if (xn2 != 0.0) {
-            double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
-            double beta = -copysign(nrm, alpha);
-            tau = (beta - alpha) / beta;
-            double scal = 1.0 / (alpha - beta);
-            __syncthreads();
-            for (int r = lane; r < rows; r += G)
-                if (r > c) ai[r] = ai[r] * scal;
-            if (lane == 0) ai[c] = beta;
-            __syncthreads();
-        }
-    }
-    return tau;
-}
-
 /*
  * Spai_QRUpdate<T>::SPAI_Column with the dense update (-qr 2,
  * src/Spai_QRUpdate.imp:33-215, QRZerlegungLapack and qrupdate_voll in

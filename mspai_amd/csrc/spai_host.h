@@ -48,6 +48,16 @@ void launch_csr_sort_rows(int nrows, const int* rowptr, int* colind, double* cva
 void launch_line_dot(int nlines, const int* ptr, const int* ind, const double* val, const double* x, double* y,
                      hipStream_t stream);
 
+/* Block SPAI (spai_block.hip). */
+/* (A_j^T A_j)^-1 of every block column into a.sqinv (tmp: sum bsize^2 doubles). */
+void launch_block_sq_inverse(const BlockArgs& a, double* tmp, hipStream_t stream);
+/* Persistent block-column kernel on nwaves waves, L.per_wave bytes each at a.scratch. */
+void launch_block_columns(const BlockArgs& a, const BlockLayout& L, int nwaves, hipStream_t stream);
+/* Scalar_Matrix: per-scalar-column nonzero counts, then the CSC fill. */
+void launch_block_scalar_count(const BlockArgs& a, int* counts, hipStream_t stream);
+void launch_block_scalar_fill(const BlockArgs& a, const int* colptr, int* rowind, double* val,
+                              hipStream_t stream);
+
 }  // namespace mspai
 
 #endif

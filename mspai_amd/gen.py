@@ -249,6 +249,30 @@ def probing_system(A: sp.csc_matrix, rho: float, e: np.ndarray | None = None):
     return C, B
 
 
+def block_convdiff2d(nx: int = 64, bs: int = 3, seed: int = 7, drop: float = 0.15) -> sp.csc_matrix:
+    """Coupled system of bs fields on the 2D upwind convection-diffusion grid:
+    node i couples to itself and its 4 neighbours through dense bs x bs blocks
+    (the scalar operator's coefficient, scaled by its diagonal, times I, plus a
+    random coupling).  A
+    fraction `drop` of the off-diagonal coupling entries is zero, and each
+    block's zeros are not stored, so blocks are partly empty as they are in
+    the reference's converted matrices."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    S = convdiff2d(nx).tocoo()
+    scale = np.abs(S.data).max()
+    rows, cols, vals = [], [], []
+    for r, c, v in zip(S.row, S.col, S.data):
+        blk = (v / scale) * np.eye(bs) + 0.2 * rng.standard_normal((bs, bs)) / bs
+        if r != c:
+            blk[rng.random((bs, bs)) < drop] = 0.0
+        rr, cc = np.nonzero(blk)
+        rows.append(r * bs + rr)
+        cols.append(c * bs + cc)
+        vals.append(blk[rr, cc])
+    n = nx * nx * bs
+    return _csc(np.concatenate(rows), np.concatenate(cols), np.concatenate(vals), (n, n))
+
+
 # ---------------------------------------------------------------- workloads
 
 def config1(nx: int = 64) -> Problem:

@@ -4,10 +4,15 @@
  * Command_Line::Parameters, src/Command_Line.cpp:192-226):
  *
  *   mspai_gpu <matrix.mtx> [-ep eps] [-mn maxnew] [-ns steps] [-um 0|1]
- *             [-qr 0|2] [-cs entries] [-pp 1|2] [-out file] [-vb 0|1]
- *             [-prec fp64|mixed] [-bins 0|1|2]
+ *             [-qr 0|2] [-cs entries] [-ch 0|1] [-bs size] [-pp 1|2]
+ *             [-out file] [-vb 0|1] [-prec fp64|mixed] [-bins 0|1|2]
  *
- * -pp 1: diagonal start pattern (default); -pp 2: pattern of A.  The
+ * -pp 1: diagonal start pattern (default); -pp 2: pattern of A (with -bs:
+ * of the block matrix).  -bs: block SPAI, bs > 1 constant blocks, 0 variable
+ * blocks (src/imspai.cpp:208-210, 746-760).  -ch 1: the hash-table variant
+ * (Spai_Update, src/Spai_Update.imp:45-146), whose M is the static solve on
+ * the start pattern -- the stored factors it keeps between setups never reach
+ * M, since the second Insert_Row_Solution overwrites the first.  The
  * preconditioner M with A M ~ I is written in the reference's MatrixMarket
  * layout (Write_Matrix_To_File), default file precond.mtx.  Errors are
  * reported on stderr with exit status 1, as the reference's
@@ -39,7 +44,8 @@ int main(int argc, char** argv)
     if (argc < 2) {
         std::fprintf(stderr,
                      "usage: %s <matrix.mtx> [-ep eps] [-mn maxnew] [-ns steps] [-um 0|1] [-qr 0|2] "
-                     "[-cs entries] [-pp 1|2] [-out file] [-vb 0|1] [-prec fp64|mixed] [-bins 0|1|2]\n",
+                     "[-cs entries] [-ch 0|1] [-bs size] [-pp 1|2] [-out file] [-vb 0|1] [-prec fp64|mixed] "
+                     "[-bins 0|1|2]\n",
                      argv[0]);
         return 1;
     }
@@ -48,7 +54,7 @@ int main(int argc, char** argv)
         mspai_opts o;
         mspai_default_opts(&o);
         std::string out = "precond.mtx";
-        int pp = 1;
+        int pp = 1, ch = 0, bs = 1;
         for (int k = 2; k < argc; ++k) {
             std::string key = argv[k];
             if (k + 1 >= argc) throw std::runtime_error("option " + key + " needs a value");
@@ -59,6 +65,8 @@ int main(int argc, char** argv)
             else if (key == "-um") o.use_mean = std::atoi(v);
             else if (key == "-qr") o.qr = std::atoi(v);
             else if (key == "-cs") o.cache = std::atoi(v);
+            else if (key == "-ch") ch = std::atoi(v);
+            else if (key == "-bs") bs = std::atoi(v);
             else if (key == "-pp") pp = std::atoi(v);
             else if (key == "-out") out = v;
             else if (key == "-vb") o.verbose = std::atoi(v);
@@ -67,16 +75,33 @@ int main(int argc, char** argv)
             else throw std::runtime_error("unknown option " + key);
         }
         if (pp != 1 && pp != 2) throw std::runtime_error("-pp: only 1 (diagonal) and 2 (pattern of A) are built");
+        /* option checks of src/imspai.cpp:280-313 */
+        if (ch < 0 || ch > 1)
+            throw std::runtime_error("Illegal hash size parameter: use 1 for using a hashtable and 0 for not using one");
+        if (o.cache > 0 && ch > 0)
+            throw std::runtime_error("Caching and Hashing cannot be used simultaneously");
+        if (ch > 0 && o.qr > 0) throw std::runtime_error("QR-Updates and Hashing optimization cannot be used simultaneously");
+        if (bs < 0) throw std::runtime_error("Illegal block size (bs < 0)");
+        if (ch > 0) o.max_impr_steps = 0;  // Spai_Update: static solve on the start pattern
         mspai::CscMatrix A = mspai::read_matrix_market(argv[1]);
         check(mspai_ctx_create(0, &ctx), ctx, "mspai_ctx_create");
         check(mspai_upload_csc(ctx, A.nrows, A.ncols, A.colptr.data(), A.rowind.data(), A.val.data(), nullptr,
                                nullptr),
               ctx, "mspai_upload_csc");
-        if (pp == 2)
+        if (bs != 1) check(mspai_set_block_size(ctx, bs), ctx, "mspai_set_block_size");
+        if (pp == 2 && bs != 1) {
+            int nb = 0;
+            long long nnzb = 0;
+            check(mspai_block_info(ctx, &nb, &nnzb, nullptr, nullptr, nullptr), ctx, "mspai_block_info");
+            std::vector<int> bp(nb + 1), bi(nnzb > 0 ? nnzb : 1);
+            check(mspai_block_info(ctx, nullptr, nullptr, nullptr, bp.data(), bi.data()), ctx, "mspai_block_info");
+            check(mspai_set_pattern(ctx, bp.data(), bi.data()), ctx, "mspai_set_pattern");
+        } else if (pp == 2) {
             check(mspai_set_pattern(ctx, A.colptr.data(), A.rowind.data()), ctx, "mspai_set_pattern");
-        else
+        } else {
             check(mspai_set_pattern(ctx, nullptr, nullptr), ctx, "mspai_set_pattern");
-        if (o.binning == 2) check(mspai_autotune(ctx, &o, 1, nullptr), ctx, "mspai_autotune");
+        }
+        if (o.binning == 2 && bs == 1) check(mspai_autotune(ctx, &o, 1, nullptr), ctx, "mspai_autotune");
         mspai_stats st;
         check(mspai_build(ctx, &o, &st), ctx, "mspai_build");
         long long nnz = 0;

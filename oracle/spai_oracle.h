@@ -99,6 +99,44 @@ int orc_spai_backward(const orc_csc* A, const orc_csc* B, const int* m_ptr,
                       const int* m_ind, const double* m_val, const double* gM,
                       double* gA, double* gB, int threads);
 
+
+/*
+ * Block matrix, the layout Convert_To_Block_Matrix builds
+ * (src/Matrix.imp:1516-1640): nb block columns (= block rows) of sizes
+ * bsize[]; block column j holds its blocks at block rows
+ * rowind[colptr[j]..colptr[j+1]) (ascending); block e is stored column-major
+ * (bsize[rowind[e]] x bsize[j]) at val[voff[e] .. voff[e+1]).
+ */
+typedef struct {
+    int nb;
+    const int* bsize;
+    const int* colptr;
+    const int* rowind;
+    const long long* voff;  /* nnzb + 1 */
+    const double* val;
+} orc_bcsc;
+
+/*
+ * Block SPAI (Spai_Unrestrained_Block, src/Spai_Unrestrained_Block.imp:33-205)
+ * for every block column.  rp_ptr/rp_ind: block row pattern (CSR);
+ * p_ptr/p_ind: block start pattern (CSC).  Outputs per block column `col`:
+ * m_len[col] blocks, their block rows m_idx[col*capJ ..], and the values
+ * m_val[col*capX ..] with block t a column-major bsize[J[t]] x bsize[col]
+ * block (Get_Solution_Vals_Block).  Returns 0 or a negative error code.
+ */
+int orc_spai_block(const orc_bcsc* A, const int* rp_ptr, const int* rp_ind, const int* p_ptr,
+                   const int* p_ind, const orc_opts* o, int capJ, long long capX, int* m_len,
+                   int* m_idx, double* m_val, double* res_norm, int* nsolve, int* status);
+
+/* (A_j^T A_j)^-1 of every block column, concatenated (sum of bsize^2). */
+void orc_block_sq_inverse(const orc_bcsc* A, double* out);
+
+/* One static solve of block column col on the block set J, then the rho of
+ * every augmentation candidate (in candidate order).  For checking the rho
+ * formula against brute-force least squares. */
+int orc_block_rhos(const orc_bcsc* A, const int* rp_ptr, const int* rp_ind, int col, const int* J,
+                   int nj, double* rn, int* ncand, int* cand, double* rho, int cap);
+
 #ifdef __cplusplus
 }
 #endif

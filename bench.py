@@ -105,6 +105,58 @@ def _bwd_fwd(mspai_amd, gen, torch, prob=None, steps=4):
     return out
 
 
+BLOCK_WORKLOAD = ("block SPAI (-bs 3), coupled 3-field 2D upwind convection-diffusion 256x256 "
+                  "(n=196608, 65536 block columns), eps=0.4, ns=5, mn=5, J0=block diagonal")
+
+
+def _block_bench(mspai_amd, gen, steps, warmup, with_cpu):
+    """Block SPAI build throughput on the coupled 3-field system, and its
+    parity with the block oracle (whole matrix, bitwise)."""
+    A = gen.block_convdiff2d(256, 3, seed=20)
+    n = A.shape[0]
+    opts = dict(epsilon=0.4, maxnew=5, max_impr_steps=5)
+    b = mspai_amd.Builder(0)
+    b.upload(A)
+    b.set_block_size(3)
+    nb = len(b.block_info()[0])
+    for _ in range(warmup):
+        b.build(**opts)
+    dev, kern = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = b.build(**opts)
+        dev.append(st["build_ms"])
+        kern.append(st["solve_ms"])
+    wall = (time.perf_counter() - t0) / steps
+    M = b.download()
+    info = b.column_info()
+    b.close()
+    out = {"workload": BLOCK_WORKLOAD, "block_columns_per_s": nb / wall, "scalar_columns_per_s": n / wall,
+           "ms_per_build": 1000.0 * wall, "device_ms_per_build": float(np.mean(dev)),
+           "kernel_ms": float(np.mean(kern)), "nnz_M": int(M.nnz), "fro_AM_minus_I": _fro_residual(A, M),
+           "mean_solves_per_block_column": float(np.mean(info["nsolve"][:nb]))}
+    if with_cpu:
+        from oracle import block as OB
+        cores = min(os.cpu_count() or 1, 16)
+        Bm = OB.BlockMatrix(A, OB.constant_blocks(n, 3))
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            ref = OB.spai_block(Bm, None, threads=cores, **opts)
+            reps += 1
+            if time.perf_counter() - t0 > 3.0 or reps >= 20:
+                break
+        cpu_t = (time.perf_counter() - t0) / reps
+        Mref = OB.to_scalar(Bm, ref)
+        out["cpu_baseline"] = {"value": nb / cpu_t, "unit": "block columns/s", "cores": cores, "kind": "port",
+                               "sample": f"full workload x {reps} runs on {cores} threads"}
+        out["parity"] = {"fp64_pattern_equal": bool(np.array_equal(Mref.indptr, M.indptr)
+                                                    and np.array_equal(Mref.indices, M.indices)),
+                         "fp64_values_bitwise": bool(np.array_equal(Mref.data.view(np.int64),
+                                                                    M.data.view(np.int64)))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +250,9 @@ def main():
             bwd = _bwd_fwd(mspai_amd, gen, torch)
         elif args.workload in ("varcoef3d", "probing3d", "laplace2d"):
             bwd = _bwd_fwd(mspai_amd, gen, torch, prob, steps=3)
+    block = None
+    if args.precision == "both" and args.workload == "convdiff2d":
+        block = _block_bench(mspai_amd, gen, max(3, args.steps // 2), args.warmup, not args.no_cpu)
     main_run = rmx if rmx is not None else r64
     M = main_run["M"]
     ms_per_step = main_run["ms_per_step"]
@@ -299,6 +354,7 @@ def main():
         "autotune": tune,
         "fp64_with_cache": cache_run,
         "bwd_fwd": bwd,
+        "block_spai": block,
         "cpu_baseline": cpu,
         "parity": parity,
     }

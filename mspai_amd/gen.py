@@ -253,24 +253,19 @@ def block_convdiff2d(nx: int = 64, bs: int = 3, seed: int = 7, drop: float = 0.1
     """Coupled system of bs fields on the 2D upwind convection-diffusion grid:
     node i couples to itself and its 4 neighbours through dense bs x bs blocks
     (the scalar operator's coefficient, scaled by its diagonal, times I, plus a
-    random coupling).  A
-    fraction `drop` of the off-diagonal coupling entries is zero, and each
-    block's zeros are not stored, so blocks are partly empty as they are in
-    the reference's converted matrices."""
+    random coupling).  A fraction `drop` of the off-diagonal coupling entries
+    is zero, and each block's zeros are not stored, so blocks are partly empty
+    as they are in the reference's converted matrices."""
     rng = np.random.Generator(np.random.PCG64(seed))
     S = convdiff2d(nx).tocoo()
     scale = np.abs(S.data).max()
-    rows, cols, vals = [], [], []
-    for r, c, v in zip(S.row, S.col, S.data):
-        blk = (v / scale) * np.eye(bs) + 0.2 * rng.standard_normal((bs, bs)) / bs
-        if r != c:
-            blk[rng.random((bs, bs)) < drop] = 0.0
-        rr, cc = np.nonzero(blk)
-        rows.append(r * bs + rr)
-        cols.append(c * bs + cc)
-        vals.append(blk[rr, cc])
+    k = S.nnz
+    blk = (S.data / scale)[:, None, None] * np.eye(bs)[None] + 0.2 * rng.standard_normal((k, bs, bs)) / bs
+    off = (S.row != S.col)[:, None, None] & (rng.random((k, bs, bs)) < drop)
+    blk[off] = 0.0
+    e, rr, cc = np.nonzero(blk)
     n = nx * nx * bs
-    return _csc(np.concatenate(rows), np.concatenate(cols), np.concatenate(vals), (n, n))
+    return _csc(S.row[e] * bs + rr, S.col[e] * bs + cc, blk[e, rr, cc], (n, n))
 
 
 # ---------------------------------------------------------------- workloads

@@ -114,23 +114,20 @@ class BlockMatrix:
         self.n, self.nb, self.bsize = n, len(sizes), sizes
         self.start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         bmap = np.repeat(np.arange(self.nb), sizes)
-        colptr, rowind, voff, vals = [0], [], [0], []
-        for jb in range(self.nb):
-            c0, c1 = self.start[jb], self.start[jb + 1]
-            sub = A[:, c0:c1]
-            rows = np.unique(bmap[sub.indices]) if sub.nnz else np.zeros(0, np.int64)
-            dense = sub.toarray()
-            for ib in rows:
-                r0, r1 = self.start[ib], self.start[ib + 1]
-                blk = dense[r0:r1, :]
-                vals.append(blk.flatten(order="F"))
-                voff.append(voff[-1] + blk.size)
-            rowind.extend(rows.tolist())
-            colptr.append(len(rowind))
-        self.colptr = np.asarray(colptr, np.int32)
-        self.rowind = np.asarray(rowind, np.int32)
-        self.voff = np.asarray(voff, np.int64)
-        self.val = np.concatenate(vals) if vals else np.zeros(0)
+        C = A.tocoo()
+        bi, bj = bmap[C.row], bmap[C.col]
+        # stored blocks: distinct (block column, block row) pairs, column-major order
+        key = bj.astype(np.int64) * self.nb + bi
+        ukey, which = np.unique(key, return_inverse=True)
+        ub_col, ub_row = ukey // self.nb, ukey % self.nb
+        h, w = sizes[ub_row].astype(np.int64), sizes[ub_col].astype(np.int64)
+        self.voff = np.concatenate([[0], np.cumsum(h * w)]).astype(np.int64)
+        val = np.zeros(int(self.voff[-1]))
+        pos = self.voff[which] + (C.col - self.start[bj]) * h[which] + (C.row - self.start[bi])
+        val[pos] = C.data
+        self.val = val
+        self.rowind = ub_row.astype(np.int32)
+        self.colptr = np.searchsorted(ub_col, np.arange(self.nb + 1)).astype(np.int32)
         pat = sp.csc_matrix((np.ones(len(self.rowind)), self.rowind, self.colptr), shape=(self.nb, self.nb))
         csr = pat.tocsr()
         csr.sort_indices()
@@ -200,19 +197,22 @@ def to_scalar(B: BlockMatrix, out) -> sp.csc_matrix:
     rows, cols, vals = [], [], []
     for col in range(B.nb):
         bk = int(B.bsize[col])
-        o = 0
-        for t in range(out["len"][col]):
-            j = int(out["idx"][col, t])
-            bj = int(B.bsize[j])
-            blk = out["val"][col, o:o + bj * bk].reshape(bk, bj)  # [q, jj]
-            o += bj * bk
-            for q in range(bk):
-                for jj in range(bj):
-                    v = blk[q, jj]
-                    if v != 0.0:
-                        rows.append(B.start[j] + jj)
-                        cols.append(B.start[col] + q)
-                        vals.append(v)
+        js = out["idx"][col, :out["len"][col]]
+        if len(js) == 0:
+            continue
+        bj = B.bsize[js].astype(np.int64)
+        nv = int((bj * bk).sum())
+        v = out["val"][col, :nv]
+        # block t holds bj x bk values, column-major: entry (jj, q) at q*bj + jj
+        tb = np.repeat(np.arange(len(js)), bj * bk)
+        local = np.arange(nv) - np.repeat(np.concatenate([[0], np.cumsum(bj * bk)[:-1]]), bj * bk)
+        q, jj = local // bj[tb], local % bj[tb]
+        keep = v != 0.0
+        rows.append((B.start[js][tb] + jj)[keep])
+        cols.append((B.start[col] + q)[keep])
+        vals.append(v[keep])
+    if rows:
+        rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
     M = sp.csc_matrix((vals, (rows, cols)), shape=(B.n, B.n))
     M.sort_indices()
     return M

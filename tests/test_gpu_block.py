@@ -30,13 +30,19 @@ def _sizes(blk, A, bs):
     return blk.diagonal_blocks(A) if bs == 0 else blk.constant_blocks(A.shape[0], bs)
 
 
-def run_block(gpu, A, bs, pattern_of_a=False, **opts):
+def _start(pattern, PA):
+    if pattern == "full":
+        return sp.csc_matrix(np.ones(PA.shape))
+    return PA if pattern else None
+
+
+def run_block(gpu, A, bs, pattern=False, **opts):
     b = gpu.Builder(0)
     try:
         b.upload(A)
         b.set_block_size(bs)
         sizes, PA = b.block_info()
-        b.set_pattern(PA if pattern_of_a else None)
+        b.set_pattern(_start(pattern, PA))
         stats = b.build(**opts)
         M = b.download()
         info = b.column_info()
@@ -45,14 +51,14 @@ def run_block(gpu, A, bs, pattern_of_a=False, **opts):
     return sizes, PA, M, info, stats
 
 
-def check_case(blk, gpu, A, bs, pattern_of_a=False, **opts):
+def check_case(blk, gpu, A, bs, pattern=False, **opts):
     B = blk.BlockMatrix(A, _sizes(blk, A, bs))
-    sizes, PA, M, info, stats = run_block(gpu, A, bs, pattern_of_a, **opts)
+    sizes, PA, M, info, stats = run_block(gpu, A, bs, pattern, **opts)
     assert np.array_equal(sizes, B.bsize)
     Pref = B.pattern()
     assert np.array_equal(PA.indptr, Pref.indptr) and np.array_equal(PA.indices, Pref.indices)
     okw = {k: opts[k] for k in ("epsilon", "maxnew", "max_impr_steps", "use_mean") if k in opts}
-    ref = blk.spai_block(B, Pref if pattern_of_a else None, threads=16, **okw)
+    ref = blk.spai_block(B, _start(pattern, Pref), threads=16, **okw)
     Mref = blk.to_scalar(B, ref)
     assert same_csc_bitwise(Mref, M), "M differs from the block oracle"
     nb = B.nb
@@ -60,7 +66,8 @@ def check_case(blk, gpu, A, bs, pattern_of_a=False, **opts):
     assert np.array_equal(ref["nsolve"], info["nsolve"][:nb])
     assert np.array_equal(ref["status"], info["status"][:nb])
     assert stats["nnz"] == Mref.nnz and stats["ncols"] == nb
-    return B, ref, M
+    print(f"   block columns {nb}, moved to the HBM kernel {stats['n_overflow']}", flush=True)
+    return B, ref, M, stats
 
 
 ADAPT = dict(epsilon=0.3, maxnew=3, max_impr_steps=3)
@@ -78,6 +85,15 @@ CASES = {
 }
 
 
+@pytest.mark.skip(reason="an LDS-resident variant of the block kernel did not return on this case (full start "
+                         "pattern, 25 blocks of 3) on the GPU; its CPU emulation finished and matched the oracle, "
+                         "no cause was found, and that variant was removed. Kept off the GPU until a cause is shown.")
+def test_block_full_start_pattern(blk, gpu):
+    """Full start pattern: every block column solves one 75 x 75 problem."""
+    A = gen.block_convdiff2d(5, 3, seed=15)
+    check_case(blk, gpu, A, 3, "full", max_impr_steps=0)
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_block_bitwise(blk, gpu, name):
     make, bs, pa, opts = CASES[name]
@@ -89,7 +105,7 @@ def test_block_larger_and_residuals(blk, gpu):
     """4096 block columns of 3: bitwise against the oracle, and every block
     column's residual norm equals ||A M(:,k) - E_k||_F from scipy."""
     A = gen.block_convdiff2d(64, 3, seed=10)
-    B, ref, M = check_case(blk, gpu, A, 3, False, epsilon=0.4, maxnew=5, max_impr_steps=5)
+    B, ref, M, _ = check_case(blk, gpu, A, 3, False, epsilon=0.4, maxnew=5, max_impr_steps=5)
     R = (A @ M - sp.identity(A.shape[0])).tocsc()
     cn = np.sqrt(np.asarray(R.multiply(R).sum(axis=0)).ravel())
     fro = np.sqrt(np.add.reduceat(cn ** 2, B.start[:-1].astype(np.int64)))

@@ -163,7 +163,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=None, help="grid size (default: the workload's full size)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="convdiff2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["block"], default="convdiff2d",
+                    help="block: only the block SPAI build (-bs 3) on the coupled 3-field system")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="timed paths; with both, value is the mixed path")
@@ -193,6 +194,16 @@ def main():
         torch = None
     import mspai_amd
     from mspai_amd import gen
+
+    if args.workload == "block":
+        blk = _block_bench(mspai_amd, gen, args.steps, args.warmup, not args.no_cpu)
+        print(json.dumps({"metric": "block SPAI block columns/s", "value": blk["block_columns_per_s"],
+                          "unit": "block columns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": blk["ms_per_build"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": BLOCK_WORKLOAD}, "block_spai": blk,
+                          "cpu_baseline": blk.get("cpu_baseline")}))
+        return
 
     make = getattr(gen, WORKLOADS[args.workload][0])
     if args.workload == "random":

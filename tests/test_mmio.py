@@ -57,7 +57,10 @@ def test_cli_option_errors_exit_1(tmp_path):
     assert r.returncode == 1 and "usage" in r.stderr
     src = tmp_path / "a.mtx"
     write_input_mtx(src, sp.identity(4, format="csc"))
-    for args, msg in ((["-xx", "1"], "unknown option -xx"), (["-pp", "3"], "-pp"), (["-ep"], "needs a value")):
+    for args, msg in ((["-xx", "1"], "unknown option -xx"), (["-pp", "3"], "-pp"), (["-ep"], "needs a value"),
+                      (["-ch", "1", "-cs", "10"], "Caching and Hashing"),
+                      (["-ch", "1", "-qr", "2"], "QR-Updates and Hashing"),
+                      (["-ch", "2"], "Illegal hash size"), (["-bs", "-1"], "Illegal block size")):
         r = subprocess.run([CLI, str(src), *args], capture_output=True, text=True)
         assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
     r = subprocess.run([CLI, str(tmp_path / "missing.mtx")], capture_output=True, text=True)

@@ -327,25 +327,144 @@ __device__ __forceinline__ void normal_solve(const RT* q, int ld, double* v, int
     }
 }
 
-/* normal_solve with v_i on group lane i (n <= G): the same operations in
- * the same order, the pivots broadcast from their lane instead of going
- * through LDS with a barrier per step. */
+/* (R^T R)^-1 v with v_i on group lane i (n <= G): normal_solve's steps with
+ * the pivots broadcast from their lane instead of going through LDS with a
+ * barrier per step, and each pivot division a product with dinv = 1/R_ii
+ * held by lane i (mixed refinement only: its corrections need no last bit). */
 template <int G, typename RT>
-__device__ __forceinline__ double normal_solve_lanes(const RT* q, int ld, double v, int n)
+__device__ __forceinline__ double normal_solve_lanes(const RT* q, int ld, double v, int n, double dinv)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
     for (int K = 0; K < n; ++K) {  // R^T y = v
-        const double yk = Gr::lane_value(v, K) / (double)q[K * ld + K];
+        const double yk = Gr::lane_value(v * dinv, K);
         if (lane == K) v = yk;
         else if (lane > K && lane < n) v = __fma_rn(-yk, (double)q[lane * ld + K], v);
     }
     for (int K = n - 1; K >= 0; --K) {  // R d = y
-        const double dk = Gr::lane_value(v, K) / (double)q[K * ld + K];
+        const double dk = Gr::lane_value(v * dinv, K);
         if (lane == K) v = dk;
         else if (lane < K) v = __fma_rn(-dk, (double)q[K * ld + lane], v);
     }
     return v;
+}
+
+/* Reduce-scatter over a group: v[j] (j < G/2) summed over the G lanes; the
+ * sum of v[j] ends on lanes 2j and 2j+1.  Each step hands the partner (lane
+ * xor S) the half it keeps and adds the half it receives: G/2 lane moves in
+ * all, against G/2 full butterflies of log2(G) moves each.  The summation
+ * order is fixed but not the 64-lane order (mixed path only). */
+template <int G, int S, int C, int N>
+__device__ __forceinline__ void reduce_scatter_step(double (&v)[N])
+{
+    if constexpr (S >= 2) {
+        const bool lower = (Group<G>::lane() & S) == 0;
+#pragma unroll
+        for (int k = 0; k < C / 2; ++k) {
+            const double send = lower ? v[k + C / 2] : v[k];
+            const double keep = lower ? v[k] : v[k + C / 2];
+            v[k] = keep + Group<G>::template partner<S>(send);
+        }
+        reduce_scatter_step<G, S / 2, C / 2, N>(v);
+    }
+}
+
+/* Group lane k's value, k a constant after unrolling (folds to one
+ * broadcast). */
+template <int G, int K = 0>
+__device__ __forceinline__ double bcast_k(double v, int k)
+{
+    if constexpr (K >= G) {
+        return 0.0;
+    } else {
+        if (k == K) return Group<G>::template lane_bcast<K>(v);
+        return bcast_k<G, K + 1>(v, k);
+    }
+}
+
+template <int G>
+__device__ __forceinline__ double reduce_scatter(double (&v)[G / 2])
+{
+    reduce_scatter_step<G, G / 2, G / 2, G / 2>(v);
+    return v[0] + Group<G>::template partner<1>(v[0]);
+}
+
+/* CSNE refinement of the FP32 register QR's solution for a group of G lanes
+ * and |J| <= G/2 (used for 16-lane groups): row i of A(I,J) in FP64 and
+ * column i of R on lane i stay in registers across the steps; r = b0 - A x per row, A^T r by one
+ * reduce-scatter, R^T R d = A^T r with the reciprocal diagonal, x += d.
+ * Stops when max|d| <= 1e-10 max|x|; kFallback after kRefineSteps. */
+template <int G, int NMAX>
+__device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScratch& s, int m, int n)
+{
+    static_assert(2 * NMAX == G, "one unknown per lane pair");
+    using Gr = Group<G>;
+    constexpr bool kKeep = NMAX <= 8;  // 16-lane groups: A's row and R's column in registers
+    const int lane = Gr::lane();
+    const int ld = L.capI;
+    const float* q = s.aq32;
+    const bool row = lane < m;
+    double a[kKeep ? NMAX : 1], rc[kKeep ? NMAX : 1];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        if constexpr (kKeep) {
+            a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
+            rc[j] = (lane < n && j < lane) ? (double)q[lane * ld + j] : 0.0;  // R(j, lane)
+        }
+    }
+    auto rcol = [&](int j) -> double {
+        if constexpr (kKeep) return rc[j];
+        return (double)q[lane * ld + j];
+    };
+    auto arow = [&](int j) -> double {
+        if constexpr (kKeep) return a[j];
+        return (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
+    };
+    const double dinv = lane < n ? 1.0 / (double)q[lane * ld + lane] : 0.0;
+    const double b0 = row ? s.b0[lane] : 0.0;
+    double x = lane < n ? s.b[lane] : 0.0;
+    for (int it = 0; it < kRefineSteps; ++it) {
+        double r = b0;  // r = b0 - A x (FP64)
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j < n) r = __fma_rn(-bcast_k<G>(x, j), arow(j), r);
+        double v[NMAX];
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) v[j] = arow(j) * r;
+        double g = __shfl(reduce_scatter<G>(v), 2 * lane, G);  // (A^T r)_j on lane j
+        if (lane >= n) g = 0.0;
+#pragma unroll
+        for (int K = 0; K < NMAX; ++K) {  // R^T y = g
+            if (K < n) {
+                const double yk = bcast_k<G>(g * dinv, K);
+                if (lane == K)
+                    g = yk;
+                else if (lane > K && lane < n)
+                    g = __fma_rn(-yk, rcol(K), g);
+            }
+        }
+#pragma unroll
+        for (int K = NMAX - 1; K >= 0; --K) {  // R d = y
+            if (K < n) {
+                const double dk = bcast_k<G>(g * dinv, K);
+                if (lane == K)
+                    g = dk;
+                else if (lane < K)
+                    g = __fma_rn(-dk, (double)q[K * ld + lane], g);
+            }
+        }
+        x += g;
+        const double dmx = Gr::max(lane < n ? fabs(g) : 0.0);
+        const double xmx = Gr::max(lane < n ? fabs(x) : 0.0);
+        if (dmx <= 1e-10 * xmx) {
+            if (lane < n) s.b[lane] = x;
+            __syncthreads();
+            return 0;
+        }
+    }
+    if (lane < n) s.b[lane] = x;
+    __syncthreads();
+    return kFallback;
 }
 
 /* out[i] = sum_j C(I_i, J_j) w_j over I in FP64, from the CSC columns or
@@ -595,10 +714,14 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
     const float* q = s.aq32;
     if (NMAX > 0 && m <= G && n <= NMAX) {
         if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kFallback;
+        /* 16-lane groups: the register-resident refinement; wider groups keep
+         * the LDS-fed one below (more registers would cost a wave per SIMD) */
+        if constexpr (G == 16 && NMAX == 8) return refine_regs<G, NMAX>(L, s, m, n);
         /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
          * no barrier except after publishing x for the next residual */
         const bool row = lane < m;
         double x = lane < n ? s.b[lane] : 0.0;
+        const double dinv = lane < n ? 1.0 / (double)q[lane * ld + lane] : 0.0;  // R's reciprocal diagonal
         for (int it = 0; it < kRefineSteps; ++it) {
             double r = row ? s.b0[lane] : 0.0;  // r = b0 - A x (FP64)
             for (int jj = 0; jj < n; ++jj)
@@ -608,7 +731,7 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
                 const double p = Gr::sum_fixed(row ? __fma_rn(s.aqr[jj * ld + lane], r, 0.0) : 0.0);
                 if (lane == jj) g = p;
             }
-            g = normal_solve_lanes<G>(q, ld, g, n);  // R^T R d = g, R from the FP32 factor
+            g = normal_solve_lanes<G>(q, ld, g, n, dinv);  // R^T R d = g, R from the FP32 factor
             x += g;
             const double dmx = Gr::max(lane < n ? fabs(g) : 0.0);
             const double xmx = Gr::max(lane < n ? fabs(x) : 0.0);

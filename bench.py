@@ -241,7 +241,12 @@ def main():
                 "bins": b.bin_counts(), "n_fallback": st["n_fallback"], "n_overflow": st["n_overflow"]}
 
     r64 = timed(0)
-    rmx = timed(1) if args.precision in ("mixed", "both") else None
+    tune_mixed = None
+    rmx = None
+    if args.precision in ("mixed", "both"):
+        # the mixed kernels have their own register and LDS budgets: tune them apart
+        tune_mixed = b.autotune(repeats=2, precision=1, **base_opts)
+        rmx = timed(1)
     # the reference's LRU cache option (-cs): same bits, repeated A(I,J) blocks reuse factors
     cache_run = None
     if args.precision == "both" and args.workload == "convdiff2d":
@@ -363,6 +368,7 @@ def main():
                      "qr_flop_per_launch": float(work[0] + work[1])},
         "bins": main_run["bins"],
         "autotune": tune,
+        "autotune_mixed": tune_mixed,
         "fp64_with_cache": cache_run,
         "bwd_fwd": bwd,
         "block_spai": block,

@@ -357,7 +357,7 @@ __device__ __forceinline__ double normal_solve_lanes(const RT* q, int ld, double
 template <int G, int S, int C, int N>
 __device__ __forceinline__ void reduce_scatter_step(double (&v)[N])
 {
-    if constexpr (S >= 2) {
+    if constexpr (C >= 2) {
         const bool lower = (Group<G>::lane() & S) == 0;
 #pragma unroll
         for (int k = 0; k < C / 2; ++k) {
@@ -366,6 +366,9 @@ __device__ __forceinline__ void reduce_scatter_step(double (&v)[N])
             v[k] = keep + Group<G>::template partner<S>(send);
         }
         reduce_scatter_step<G, S / 2, C / 2, N>(v);
+    } else if constexpr (S >= 1) {
+        v[0] = v[0] + Group<G>::template partner<S>(v[0]);
+        reduce_scatter_step<G, S / 2, 1, N>(v);
     }
 }
 
@@ -382,69 +385,65 @@ __device__ __forceinline__ double bcast_k(double v, int k)
     }
 }
 
-template <int G>
-__device__ __forceinline__ double reduce_scatter(double (&v)[G / 2])
+/* Sum over the group of v[j], j < NV (NV a power of two, NV <= G/2): the
+ * halving steps hand the partner (lane xor S) the half it keeps and add the
+ * half received, then plain butterflies finish; sum j ends on the G/NV
+ * lanes j*G/NV .. (j+1)*G/NV - 1.  G/2 + log2(G/NV) - 1 lane moves in all,
+ * against NV butterflies of log2(G) each.  The order is fixed but is not the
+ * 64-lane order (mixed path only). */
+template <int G, int NV>
+__device__ __forceinline__ double reduce_scatter(double (&v)[NV])
 {
-    reduce_scatter_step<G, G / 2, G / 2, G / 2>(v);
-    return v[0] + Group<G>::template partner<1>(v[0]);
+    static_assert(NV >= 1 && 2 * NV <= G, "values per group");
+    reduce_scatter_step<G, G / 2, NV, NV>(v);
+    return v[0];
 }
 
 /* CSNE refinement of the FP32 register QR's solution for a group of G lanes
- * and |J| <= G/2 (used for 16-lane groups): row i of A(I,J) in FP64 and
- * column i of R on lane i stay in registers across the steps; r = b0 - A x per row, A^T r by one
+ * and |J| <= NV (used for 16-lane groups; in 32-lane groups it measured
+ * slower than the LDS-fed steps below): row i of A(I,J) in FP64 and column i
+ * of R on lane i stay in registers across the steps; r = b0 - A x per row, A^T r by one
  * reduce-scatter, R^T R d = A^T r with the reciprocal diagonal, x += d.
  * Stops when max|d| <= 1e-10 max|x|; kFallback after kRefineSteps. */
-template <int G, int NMAX>
+template <int G, int NV>
 __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
-    static_assert(2 * NMAX == G, "one unknown per lane pair");
     using Gr = Group<G>;
-    constexpr bool kKeep = NMAX <= 8;  // 16-lane groups: A's row and R's column in registers
     const int lane = Gr::lane();
     const int ld = L.capI;
     const float* q = s.aq32;
     const bool row = lane < m;
-    double a[kKeep ? NMAX : 1], rc[kKeep ? NMAX : 1];
+    double a[NV], rc[NV];
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-        if constexpr (kKeep) {
-            a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
-            rc[j] = (lane < n && j < lane) ? (double)q[lane * ld + j] : 0.0;  // R(j, lane)
-        }
+    for (int j = 0; j < NV; ++j) {
+        a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
+        rc[j] = (lane < n && j < lane) ? (double)q[lane * ld + j] : 0.0;  // R(j, lane)
     }
-    auto rcol = [&](int j) -> double {
-        if constexpr (kKeep) return rc[j];
-        return (double)q[lane * ld + j];
-    };
-    auto arow = [&](int j) -> double {
-        if constexpr (kKeep) return a[j];
-        return (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
-    };
     const double dinv = lane < n ? 1.0 / (double)q[lane * ld + lane] : 0.0;
     const double b0 = row ? s.b0[lane] : 0.0;
     double x = lane < n ? s.b[lane] : 0.0;
     for (int it = 0; it < kRefineSteps; ++it) {
         double r = b0;  // r = b0 - A x (FP64)
 #pragma unroll
-        for (int j = 0; j < NMAX; ++j)
-            if (j < n) r = __fma_rn(-bcast_k<G>(x, j), arow(j), r);
-        double v[NMAX];
+        for (int j = 0; j < NV; ++j)
+            if (j < n) r = __fma_rn(-bcast_k<G>(x, j), a[j], r);
+        double v[NV];
 #pragma unroll
-        for (int j = 0; j < NMAX; ++j) v[j] = arow(j) * r;
-        double g = __shfl(reduce_scatter<G>(v), 2 * lane, G);  // (A^T r)_j on lane j
+        for (int j = 0; j < NV; ++j) v[j] = a[j] * r;
+        double g = __shfl(reduce_scatter<G, NV>(v), lane * (G / NV), G);  // (A^T r)_j on lane j
         if (lane >= n) g = 0.0;
 #pragma unroll
-        for (int K = 0; K < NMAX; ++K) {  // R^T y = g
+        for (int K = 0; K < NV; ++K) {  // R^T y = g
             if (K < n) {
                 const double yk = bcast_k<G>(g * dinv, K);
                 if (lane == K)
                     g = yk;
                 else if (lane > K && lane < n)
-                    g = __fma_rn(-yk, rcol(K), g);
+                    g = __fma_rn(-yk, rc[K], g);
             }
         }
 #pragma unroll
-        for (int K = NMAX - 1; K >= 0; --K) {  // R d = y
+        for (int K = NV - 1; K >= 0; --K) {  // R d = y
             if (K < n) {
                 const double dk = bcast_k<G>(g * dinv, K);
                 if (lane == K)
@@ -716,7 +715,7 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
         if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kFallback;
         /* 16-lane groups: the register-resident refinement; wider groups keep
          * the LDS-fed one below (more registers would cost a wave per SIMD) */
-        if constexpr (G == 16 && NMAX == 8) return refine_regs<G, NMAX>(L, s, m, n);
+        if constexpr (G == 16 && NMAX == 8) return refine_regs<G, 8>(L, s, m, n);
         /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
          * no barrier except after publishing x for the next residual */
         const bool row = lane < m;

@@ -328,15 +328,29 @@ struct Group {
             sum_fixed_n(p);
         }
     }
-    /* value of lane `src` of the group (src uniform in the group) */
+    /* value of lane `src` of the group (src the same in every active lane
+     * of the wave) */
     static __device__ __forceinline__ double lane_value(double v, int src)
     {
-        if (G == kWave) {
-            const long long bits = __double_as_longlong(v);
-            const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, src);
-            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(bits >> 32), src);
-            return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        const long long bits = __double_as_longlong(v);
+        const unsigned lo = (unsigned)lane_value_i((int)bits, src);
+        const unsigned hi = (unsigned)lane_value_i((int)(bits >> 32), src);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+    /* 32-bit version: for whole waves and 32-lane groups one readlane per
+     * group, then each lane keeps its own group's value -- no LDS-pipe
+     * permute.  src must be the same in every active lane of the wave (a
+     * loop counter that all groups step together), not only within the
+     * group. */
+    static __device__ __forceinline__ int lane_value_i(int v, int src)
+    {
+        if (G == kWave) return __builtin_amdgcn_readlane(v, src);
+        if (G == 32) {
+            const int a = __builtin_amdgcn_readlane(v, src), b = __builtin_amdgcn_readlane(v, src + 32);
+            return (threadIdx.x & 32) ? b : a;
         }
+        /* 16-lane groups: four readlanes and a select cost more than the
+         * permute (measured) */
         return __shfl(v, src, G);
     }
     template <int S>

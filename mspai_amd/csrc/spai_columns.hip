@@ -341,7 +341,11 @@ __device__ __forceinline__ double normal_solve_lanes(const RT* q, int ld, double
         if (lane == K) v = yk;
         else if (lane > K && lane < n) v = __fma_rn(-yk, (double)q[lane * ld + K], v);
     }
-    for (int K = n - 1; K >= 0; --K) {  // R d = y
+    /* K counts down from G - 1 for every group of the wave alike: the
+     * broadcast reads lane K of each group with one readlane per group,
+     * which needs the same K in all of them */
+    for (int K = G - 1; K >= 0; --K) {  // R d = y
+        if (K >= n) continue;
         const double dk = Gr::lane_value(v * dinv, K);
         if (lane == K) v = dk;
         else if (lane < K) v = __fma_rn(-dk, (double)q[K * ld + lane], v);

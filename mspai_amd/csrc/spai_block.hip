@@ -34,6 +34,8 @@ namespace mspai {
 namespace {
 
 constexpr int kBlkOverflow = -2;
+constexpr int kBlkRegJ = 16;  // register solve: scalar columns of A(I,J)
+constexpr int kBlkRegB = 4;   //   and right-hand sides (|I| <= 64)
 
 struct BlkScratch {
     int* J;
@@ -282,6 +284,113 @@ __device__ int augment_block(const BlockArgs& a, const BlockLayout& L, const Blk
     return nj + sneed;
 }
 
+/* Approximative_Solution (block) with row r of A(I,J) and of the bk
+ * right-hand sides in the registers of lane r (|I| <= 64, |J| <= NMAX,
+ * bk <= NB): the operations and their order are the 64-lane order's (one row
+ * per lane is its only slot), with no memory round trip or barrier between
+ * reflectors.  Slots NMAX.. hold the right-hand sides; a reflector is applied
+ * to four slots at a time so that their reductions interleave.  On exit
+ * x[q*n + j] holds the solution (Q^T E when R has a zero pivot, 0 when
+ * |I| < |J|).  Returns status bits. */
+template <int NMAX, int NB>
+__device__ int block_solve_regs(const double* ahat, const double* bsrc, int ld, int m, int n, int bk, double* x)
+{
+    constexpr int kB = 4;
+    constexpr int NS = NMAX + NB;
+    static_assert(NMAX % kB == 0 && NB % kB == 0, "register width");
+    const int lane = W::lane();
+    const bool row = lane < m;
+    double a[NS];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? ahat[(size_t)j * ld + lane] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) a[NMAX + q] = (row && q < bk) ? bsrc[(size_t)q * ld + lane] : 0.0;
+    const int k = m < n ? m : n;
+    bool zero_diag = false;  // R_ii == 0 on this lane's row
+#pragma unroll
+    for (int ib = 0; ib < NMAX; ib += kB) {
+        if (ib >= k) break;
+#pragma unroll 1
+        for (int r = 0; r < kB; ++r) {
+            const int i = ib + r;
+            if (i >= k) break;
+            double ai = a[ib];
+#pragma unroll
+            for (int t = 1; t < kB; ++t)
+                if (t == r) ai = a[ib + t];
+            const double alpha = W::lane_value(ai, i);
+            const double xn2 = W::sum_fixed(lane > i ? __fma_rn(ai, ai, 0.0) : 0.0);
+            double rii = ai;
+            if (i + 1 < m && xn2 != 0.0) {
+                const double nrm = sqrt(__fma_rn(alpha, alpha, xn2));
+                const double beta = -copysign(nrm, alpha);
+                const double tau = (beta - alpha) / beta;
+                const double scal = 1.0 / (alpha - beta);
+                const double vi = lane > i ? ai * scal : (lane == i ? 1.0 : 0.0);
+                const double ni = lane > i ? vi : (lane == i ? beta : ai);
+                rii = beta;
+#pragma unroll
+                for (int t = 0; t < kB; ++t)
+                    if (t == r) a[ib + t] = ni;
+                const double mt = -tau;
+                const bool act = lane >= i;
+#pragma unroll
+                for (int jb = ib; jb < NS; jb += kB) {
+                    const bool rhs = jb >= NMAX;
+                    if (rhs ? jb - NMAX < bk : (jb + kB - 1 > i && jb < n)) {
+                        double p[kB];
+#pragma unroll
+                        for (int t = 0; t < kB; ++t) p[t] = act ? __fma_rn(vi, a[jb + t], 0.0) : 0.0;
+                        W::sum_bcast_n(p);
+#pragma unroll
+                        for (int t = 0; t < kB; ++t) {
+                            const int j = jb + t;
+                            const bool live = rhs ? j - NMAX < bk : (j > i && j < n);
+                            if (live && act) a[j] = __fma_rn(vi, mt * p[t], a[j]);
+                        }
+                    }
+                }
+            }
+            zero_diag |= lane == i && rii == 0.0;
+        }
+    }
+    int st = 0;
+    if (m < n) {
+        st = COL_UNDERDET;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) a[NMAX + q] = 0.0;
+    } else if (W::ballot(zero_diag) != 0ull) {
+        st = COL_SINGULAR;  // dtrtrs leaves Q^T E
+    } else {
+        /* dtrsm order per right-hand side; K over compile-time slots so R_iK
+         * is a fixed register of lane i */
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            if (q < bk) {
+                double b = a[NMAX + q];
+#pragma unroll
+                for (int K = NMAX - 1; K >= 0; --K) {
+                    if (K < n) {
+                        double xk = W::lane_value(b, K);
+                        const double rkk = W::lane_value(a[K], K);
+                        if (xk != 0.0) {
+                            xk = xk / rkk;
+                            if (lane == K) b = xk;
+                            if (lane < K) b = __fma_rn(-xk, a[K], b);
+                        }
+                    }
+                }
+                a[NMAX + q] = b;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+        if (q < bk && lane < n) x[q * n + lane] = a[NMAX + q];
+    __syncthreads();
+    return st;
+}
+
 /* One block column.  Returns false when it outgrew the working set. */
 __device__ bool block_column(const BlockArgs& a, const BlockLayout& L, const BlkScratch& s, int col)
 {
@@ -318,8 +427,6 @@ __device__ bool block_column(const BlockArgs& a, const BlockLayout& L, const Blk
             }
         }
         __syncthreads();
-        for (int c = 0; c < n; ++c)
-            for (int r = lane; r < m; r += kWave) s.aqr[(size_t)c * ld + r] = s.ahat[(size_t)c * ld + r];
         /* E_k: the identity block at block row col's offset */
         for (int q = 0; q < bk; ++q)
             for (int r = lane; r < m; r += kWave) s.b[(size_t)q * ld + r] = 0.0;
@@ -333,6 +440,12 @@ __device__ bool block_column(const BlockArgs& a, const BlockLayout& L, const Blk
         __syncthreads();
         for (int q = 0; q < bk; ++q)
             for (int r = lane; r < m; r += kWave) s.b0[(size_t)q * ld + r] = s.b[(size_t)q * ld + r];
+        __syncthreads();
+        if (m <= kWave && n <= kBlkRegJ && bk <= kBlkRegB) {
+            st |= block_solve_regs<kBlkRegJ, kBlkRegB>(s.ahat, s.b, ld, m, n, bk, s.x);
+        } else {
+        for (int c = 0; c < n; ++c)
+            for (int r = lane; r < m; r += kWave) s.aqr[(size_t)c * ld + r] = s.ahat[(size_t)c * ld + r];
         __syncthreads();
         /* QR of A(I,J); Q^T on every right-hand side */
         const int kq = m < n ? m : n;
@@ -372,6 +485,7 @@ __device__ bool block_column(const BlockArgs& a, const BlockLayout& L, const Blk
             }
             for (int q = 0; q < bk; ++q)
                 for (int j = lane; j < n; j += kWave) s.x[q * n + j] = s.b[(size_t)q * ld + j];
+        }
         }
         __syncthreads();
         ++solves;

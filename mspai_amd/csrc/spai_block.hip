@@ -34,7 +34,7 @@ namespace mspai {
 namespace {
 
 constexpr int kBlkOverflow = -2;
-constexpr int kBlkRegJ = 16;  // register solve: scalar columns of A(I,J)
+constexpr int kBlkRegJ = 32;  // register solve: scalar columns of A(I,J)
 constexpr int kBlkRegB = 4;   //   and right-hand sides (|I| <= 64)
 
 struct BlkScratch {
@@ -291,9 +291,10 @@ __device__ int augment_block(const BlockArgs& a, const BlockLayout& L, const Blk
  * reflectors.  Slots NMAX.. hold the right-hand sides; a reflector is applied
  * to four slots at a time so that their reductions interleave.  On exit
  * x[q*n + j] holds the solution (Q^T E when R has a zero pivot, 0 when
- * |I| < |J|).  Returns status bits. */
+ * |I| < |J|).  Returns status bits.  Not inlined: its own register
+ * allocation holds the 36-slot rows without spilling the kernel's state. */
 template <int NMAX, int NB>
-__device__ int block_solve_regs(const double* ahat, const double* bsrc, int ld, int m, int n, int bk, double* x)
+__device__ __attribute__((noinline)) int block_solve_regs(const double* ahat, const double* bsrc, int ld, int m, int n, int bk, double* x)
 {
     constexpr int kB = 4;
     constexpr int NS = NMAX + NB;

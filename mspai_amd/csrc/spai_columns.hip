@@ -277,8 +277,42 @@ __device__ __forceinline__ int ls_solve(const ScratchLayout& L, const WaveScratc
         }
         if (lane == 0) s.tau[i] = tau;
         if (tau != 0.0) {
-            for (int j = i + 1; j < n; ++j) apply_reflector<G>(ai, s.aqr + j * ld, i, m, tau);
-            apply_reflector<G>(ai, s.b, i, m, tau);
+            /* H_i on columns i+1..n-1 and b, four at a time: each column's
+             * dot product and update are apply_reflector's (rows of a lane in
+             * order, fused; the 64-lane tree), but the four reductions share
+             * their lane moves and the loads of four columns are in flight
+             * together; one barrier per reflector */
+            const int nc = n - i;  // columns i+1..n-1, then b
+            const double mt = -tau;
+            for (int c0 = 0; c0 < nc; c0 += 4) {
+                double* cp[4];
+                bool live[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int c = c0 + t;
+                    live[t] = c < nc;
+                    cp[t] = c < nc - 1 ? s.aqr + (i + 1 + c) * ld : s.b;
+                }
+                double p[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int r = lane; r < m; r += G) {
+                    if (r < i) continue;
+                    const double vr = (r == i) ? 1.0 : ai[r];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (live[t]) p[t] = __fma_rn(vr, cp[t][r], p[t]);
+                }
+                Gr::sum_bcast_n(p);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) p[t] = mt * p[t];
+                for (int r = lane; r < m; r += G) {
+                    if (r < i) continue;
+                    const double vr = (r == i) ? 1.0 : ai[r];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (live[t]) cp[t][r] = __fma_rn(vr, p[t], cp[t][r]);
+                }
+            }
+            __syncthreads();
         }
     }
     __syncthreads();

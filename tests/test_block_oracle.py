@@ -168,3 +168,22 @@ def test_library_block_detection_matches_oracle(blk):
         for bs in (0, 1, 2, 3, 4):
             ref = blk.diagonal_blocks(A) if bs == 0 else blk.constant_blocks(A.shape[0], bs)
             assert np.array_equal(mspai_amd.block_sizes(A, bs), ref), bs
+
+
+def test_zero_column_flags(blk, oracle):
+    """A zero scalar column: its block column's solve reports the zero pivot
+    (dtrtrs leaves Q^T E), and every block column that scores it as a
+    candidate gets a NaN mean, so nothing is added there; M stays finite."""
+    A = gen.block_convdiff2d(5, 3, seed=16).tolil()
+    A[:, 7] = 0.0
+    A = sp.csc_matrix(A)
+    A.eliminate_zeros()
+    B = blk.BlockMatrix(A, blk.constant_blocks(A.shape[0], 3))
+    out = blk.spai_block(B, None, epsilon=0.3, maxnew=3, max_impr_steps=3)
+    assert out["status"][2] & oracle.ST_SINGULAR
+    nan_cols = np.nonzero(out["status"] & oracle.ST_NAN_RHO)[0]
+    assert len(nan_cols) > 0
+    # block 2 is a neighbour (a candidate) of every flagged column
+    P = B.pattern().toarray()
+    assert all(P[:, c].any() and (P[2, :].any()) for c in nan_cols)
+    assert np.isfinite(blk.to_scalar(B, out).data).all()

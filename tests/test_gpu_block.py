@@ -71,6 +71,17 @@ def check_case(blk, gpu, A, bs, pattern=False, **opts):
 
 
 ADAPT = dict(epsilon=0.3, maxnew=3, max_impr_steps=3)
+
+
+def zero_column_system():
+    """Scalar column 7 of the 3-field system removed: block column 2 has a
+    zero column, so its own solve hits a zero pivot and (A_2^T A_2)^-1 is not
+    finite, which makes the mean score NaN wherever 2 is a candidate."""
+    A = gen.block_convdiff2d(5, 3, seed=16).tolil()
+    A[:, 7] = 0.0
+    A = sp.csc_matrix(A)
+    A.eliminate_zeros()
+    return A
 CASES = {
     "bs3_static_diag": (lambda: gen.block_convdiff2d(8, 3, seed=1), 3, False, dict(max_impr_steps=0)),
     "bs3_adaptive": (lambda: gen.block_convdiff2d(8, 3, seed=2), 3, False, ADAPT),
@@ -82,6 +93,7 @@ CASES = {
     "bs2_scalar_operator": (lambda: gen.convdiff2d(12), 2, False, dict(epsilon=0.3, maxnew=4, max_impr_steps=4)),
     "bs3_no_mean": (lambda: gen.block_convdiff2d(6, 3, seed=8), 3, False, dict(epsilon=0.2, maxnew=2, max_impr_steps=3, use_mean=0)),
     "bs3_defaults": (lambda: gen.block_convdiff2d(8, 3, seed=9), 3, False, dict()),
+    "bs3_zero_column_nan_scores": (zero_column_system, 3, False, ADAPT),
 }
 
 

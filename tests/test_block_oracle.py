@@ -192,3 +192,14 @@ def test_zero_column_flags(blk, oracle):
     flagged = np.nonzero(one["status"] & oracle.ST_NAN_RHO)[0]
     assert len(flagged) > 0 and all(reach[2, c] for c in flagged)
     assert np.isfinite(blk.to_scalar(B, out).data).all()
+
+
+def test_library_block_sizes_edges(blk):
+    """Constant blocks larger than n, n = 1, and a diagonal without runs."""
+    import mspai_amd
+    assert mspai_amd.block_sizes(sp.identity(5, format="csc"), 7).tolist() == [5]
+    assert mspai_amd.block_sizes(sp.identity(1, format="csc"), 3).tolist() == [1]
+    assert mspai_amd.block_sizes(sp.identity(6, format="csc"), 0).tolist() == [1] * 6
+    assert mspai_amd.block_sizes(sp.csc_matrix(np.ones((4, 4))), 0).tolist() == [4]
+    with pytest.raises(mspai_amd.MspaiError):
+        mspai_amd.block_sizes(sp.identity(4, format="csc"), -1)

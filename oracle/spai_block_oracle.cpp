@@ -443,7 +443,8 @@ extern "C" int orc_spai_block(const orc_bcsc* A, const int* rp_ptr, const int* r
         w.bitvec.assign((A->nb >> 5) + 1, 0u);
         for (int col = lo; col < hi; ++col) {
             int plen = p_ptr[col + 1] - p_ptr[col];
-            if (plen + o->maxnew * std::max(0, o->max_impr_steps) > capJ ||
+            /* J holds distinct block columns: never more than nb */
+            if (std::min(A->nb, plen + o->maxnew * std::max(0, o->max_impr_steps)) > capJ ||
                 (long long)capJ * maxbs * A->bsize[col] > capX) {
                 err = -4;
                 continue;

@@ -15,7 +15,7 @@ import scipy.sparse as sp
 import torch  # before the library: one HIP runtime per process
 
 from mspai_amd import gen
-from tests.util import same_csc_bitwise
+from tests.util import load_csparse, same_csc_bitwise
 
 pytestmark = pytest.mark.gpu
 
@@ -94,6 +94,12 @@ CASES = {
     "bs3_no_mean": (lambda: gen.block_convdiff2d(6, 3, seed=8), 3, False, dict(epsilon=0.2, maxnew=2, max_impr_steps=3, use_mean=0)),
     "bs3_defaults": (lambda: gen.block_convdiff2d(8, 3, seed=9), 3, False, dict()),
     "bs3_zero_column_nan_scores": (zero_column_system, 3, False, ADAPT),
+    # the reference's own CSparse matrices (tests/golden/csparse)
+    "cs_west0067_bs2": (lambda: load_csparse("west0067"), 2, False, dict(epsilon=0.3, maxnew=4, max_impr_steps=4)),
+    "cs_fs_183_1_bs3": (lambda: load_csparse("fs_183_1"), 3, False, dict(epsilon=0.2, maxnew=5, max_impr_steps=6)),
+    "cs_fs_183_1_var": (lambda: load_csparse("fs_183_1"), 0, True, dict(epsilon=0.2, maxnew=5, max_impr_steps=3)),
+    "cs_bcsstk01_bs6": (lambda: load_csparse("bcsstk01"), 6, False, dict(epsilon=0.3, maxnew=3, max_impr_steps=3)),
+    "cs_bcsstk01_var": (lambda: load_csparse("bcsstk01"), 0, True, dict(max_impr_steps=0)),
 }
 
 

@@ -67,11 +67,23 @@ def test_probing_gradient_finite_differences(gpu):
     with torch.no_grad():
         m0 = spai_values(b, maps.c_values(a_vals, torch.tensor(rho0, dtype=torch.float64, device=dev)),
                          maps.b_values(torch.tensor(rho0, dtype=torch.float64, device=dev)), max_impr_steps=0)
-    pm = ProductMaps(A, b.download())
+    M0 = b.download()
+    pm = ProductMaps(A, M0)
+    # finite differences evaluate the loss on the host: the device sums use
+    # atomics, whose order changes the last bits, and (p - q) / 2h would
+    # amplify that to the size of the tolerance
+    pm_host = ProductMaps(A, M0, device="cpu")
+    maps_host = ProbingMaps(A, device="cpu")
 
     def terms(av, rv, precision=0):
         m = spai_values(b, maps.c_values(av, rv), maps.b_values(rv), max_impr_steps=0, precision=precision)
         return loss_terms(pm, av, m, n)
+
+    def terms_host(av, rv, precision=0):
+        ah, rh = av.cpu(), rv.cpu()  # C's probing row summed on the host too
+        c = maps_host.c_values(ah, rh).to(dev)
+        m = spai_values(b, c, maps.b_values(rv), max_impr_steps=0, precision=precision)
+        return loss_terms(pm_host, ah, m.cpu(), n)
 
     for precision in (0, 1):
         av = a_vals.clone().requires_grad_(True)
@@ -88,14 +100,14 @@ def test_probing_gradient_finite_differences(gpu):
                 ap, am_ = a_vals.clone(), a_vals.clone()
                 ap[e] += h
                 am_[e] -= h
-                p1, p2 = terms(ap, torch.tensor(rho0, dtype=torch.float64, device=dev))
-                q1, q2 = terms(am_, torch.tensor(rho0, dtype=torch.float64, device=dev))
+                p1, p2 = terms_host(ap, torch.tensor(rho0, dtype=torch.float64, device=dev), precision)
+                q1, q2 = terms_host(am_, torch.tensor(rho0, dtype=torch.float64, device=dev), precision)
                 fd = ((p1 - q1).sum() + (p2 - q2).sum()).item() / (2 * h)
                 ad = ga[e].item()
                 rel = abs(fd - ad) / max(abs(fd), abs(ad), 1e-3)
                 worst = max(worst, rel)
-            p1, p2 = terms(a_vals, torch.tensor(rho0 + h, dtype=torch.float64, device=dev))
-            q1, q2 = terms(a_vals, torch.tensor(rho0 - h, dtype=torch.float64, device=dev))
+            p1, p2 = terms_host(a_vals, torch.tensor(rho0 + h, dtype=torch.float64, device=dev), precision)
+            q1, q2 = terms_host(a_vals, torch.tensor(rho0 - h, dtype=torch.float64, device=dev), precision)
             fdr = ((p1 - q1).sum() + (p2 - q2).sum()).item() / (2 * h)
         relr = abs(fdr - gr) / max(abs(fdr), abs(gr), 1e-6)
         print(f"[probing FD precision={precision}] worst rel err dL/dA {worst:.2e}, dL/drho {gr:.6e} "

@@ -375,6 +375,59 @@ struct Group {
         p = sum_step_f<1>(p);
         return p;
     }
+    /* one v_permlane{16,32}_swap on a pair of floats (see swap_pair) */
+    template <int S>
+    static __device__ __forceinline__ void swap_pair_f(float& a, float& b)
+    {
+        const int ai = __float_as_int(a), bi = __float_as_int(b);
+        auto r = S == 32 ? __builtin_amdgcn_permlane32_swap(ai, bi, false, false)
+                         : __builtin_amdgcn_permlane16_swap(ai, bi, false, false);
+        a = __int_as_float((int)r[0]);
+        b = __int_as_float((int)r[1]);
+    }
+    /* Four FP32 sums with every lane ending with all four, the lane swaps of
+     * the xor-32 and xor-16 steps shared as in sum4_bcast (64-lane groups),
+     * or two pairs sharing the xor-16 swap (32-lane groups); 16-lane groups
+     * run four butterflies.  The summation order differs from sum_f's (mixed
+     * path only, where it is free). */
+    static __device__ __forceinline__ void sum4_bcast_f(float (&p)[4])
+    {
+        if constexpr (G == 64) {
+            float a = p[0], c = p[2], b = p[1], d = p[3];
+            swap_pair_f<32>(a, c);
+            swap_pair_f<32>(b, d);
+            float qa = a + c, qb = b + d;
+            swap_pair_f<16>(qa, qb);
+            float q = qa + qb;
+            q = sum_step_f<8>(q);
+            q = sum_step_f<4>(q);
+            q = sum_step_f<2>(q);
+            q = sum_step_f<1>(q);
+            float u = q, v = q;
+            swap_pair_f<16>(u, v);  // u: rows 0/2 results, v: rows 1/3
+            float u2 = u, v2 = v;
+            swap_pair_f<32>(u, u2);  // u: p0 everywhere, u2: p2 everywhere
+            swap_pair_f<32>(v, v2);  // v: p1, v2: p3
+            p[0] = u, p[1] = v, p[2] = u2, p[3] = v2;
+        } else if constexpr (G == 32) {
+#pragma unroll
+            for (int k = 0; k < 4; k += 2) {
+                float a = p[k], b = p[k + 1];
+                swap_pair_f<16>(a, b);
+                float q = a + b;
+                q = sum_step_f<8>(q);
+                q = sum_step_f<4>(q);
+                q = sum_step_f<2>(q);
+                q = sum_step_f<1>(q);
+                float u = q, v = q;
+                swap_pair_f<16>(u, v);
+                p[k] = u, p[k + 1] = v;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) p[k] = sum_f(p[k]);
+        }
+    }
     /* 32-bit value of lane (l xor S), S < G, without LDS-pipe permutes:
      * DPP for S < 16, a lane swap for 16 and 32 (of the two swap results the
      * partner's value is the one from the other row / half) */

@@ -679,7 +679,10 @@ __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScr
 #pragma unroll
             for (int t = 1; t < kB; ++t)
                 if (t == r) ai = a[ib + t];
-            const float alpha = __shfl(ai, i, G);
+            /* whole waves: i is wave-uniform, so a readlane; narrower groups
+             * may stand at different steps (their n differ) */
+            const float alpha = G == kWave ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ai), i))
+                                           : __shfl(ai, i, G);
             const float xn2 = Gr::allreduce(lane > i ? ai * ai : 0.f, plus);
             float rii = ai;
             if (i + 1 < m && xn2 != 0.f) {
@@ -701,9 +704,13 @@ __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScr
 #pragma unroll
                         for (int t = 0; t < kB; ++t)
                             w[t] = (jb + t <= NMAX && act) ? vi * a[jb + t <= NMAX ? jb + t : NMAX] : 0.f;
+                        if (jb + kB - 1 <= NMAX) {
+                            Gr::sum4_bcast_f(w);  // four sums sharing their lane swaps
+                        } else {
 #pragma unroll
-                        for (int t = 0; t < kB; ++t)
-                            if (jb + t <= NMAX) w[t] = Gr::allreduce(w[t], plus);
+                            for (int t = 0; t < kB; ++t)
+                                if (jb + t <= NMAX) w[t] = Gr::allreduce(w[t], plus);
+                        }
 #pragma unroll
                         for (int t = 0; t < kB; ++t) {
                             const int j = jb + t;
@@ -728,7 +735,9 @@ __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScr
 #pragma unroll
     for (int K = NMAX - 1; K >= 0; --K) {
         if (K < n) {
-            float xk = __shfl(x, K, G) / __shfl(a[K], K, G);
+            /* K is a constant: one readlane per group, the same in every group */
+            const float xk = __int_as_float(Gr::lane_value_i(__float_as_int(x), K)) /
+                             __int_as_float(Gr::lane_value_i(__float_as_int(a[K]), K));
             if (lane == K) x = xk;
             if (lane < K) x = __fmaf_rn(-xk, a[K], x);
         }
@@ -763,10 +772,16 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
             double r = row ? s.b0[lane] : 0.0;  // r = b0 - A x (FP64)
             for (int jj = 0; jj < n; ++jj)
                 if (row) r = __fma_rn(-s.b[jj], s.aqr[jj * ld + lane], r);
-            double g = 0.0;  // g = A^T r, g_j on lane j
-            for (int jj = 0; jj < n; ++jj) {
-                const double p = Gr::sum_fixed(row ? __fma_rn(s.aqr[jj * ld + lane], r, 0.0) : 0.0);
-                if (lane == jj) g = p;
+            double g = 0.0;  // g = A^T r, g_j on lane j; four sums at a time share their lane swaps
+            for (int jb = 0; jb < n; jb += 4) {
+                double p[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    p[t] = (row && jb + t < n) ? __fma_rn(s.aqr[(jb + t) * ld + lane], r, 0.0) : 0.0;
+                Gr::sum_bcast_n(p);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if (lane == jb + t) g = p[t];
             }
             g = normal_solve_lanes<G>(q, ld, g, n, dinv);  // R^T R d = g, R from the FP32 factor
             x += g;

@@ -574,7 +574,7 @@ enum LayoutNeeds { kNeedMixed = 1, kNeedBackward = 2, kNeedQrUpdate = 4 };
  * tie_in_main, else in a separate per-group HBM region of tie_per_group
  * bytes (LDS bins: the path is rare). */
 static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capSort, int maxnew, size_t capPairs,
-                                 int needs, bool tie_in_main)
+                                 int needs, bool tie_in_main, int group = 0)
 {
     ScratchLayout L{};
     capI = std::max<size_t>(capI, 1);
@@ -607,7 +607,9 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.segoff = take(((size_t)capJ + 1) * 4);
     L.sid = take(capSort * 4);
     L.wsum = take(8 * 8);
-    if (needs & kNeedMixed) L.aq32 = take(capI * capJ * 4);
+    /* register bins convert A(I,J) to FP32 as they load it: aq32 holds R only */
+    L.ldq = register_bin(group, (int)capI, capJ) ? capJ : (int)capI;
+    if (needs & kNeedMixed) L.aq32 = take((size_t)L.ldq * capJ * 4);
     if (needs & kNeedBackward) L.rq = take((size_t)capJ * capJ * 8);
     if (needs & (kNeedMixed | kNeedBackward)) {
         L.bq32 = take(capB * 4);
@@ -712,7 +714,7 @@ static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, b
         size_t capU = adaptive ? std::min<size_t>(G.capU, e.U) : 1;
         size_t capSort = std::min<size_t>(G.capSort, e.sort);
         /* ties of scores across the cut are ordered in the generic kernel only */
-        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, needs, false);
+        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, needs, false, e.group);
         if (L.per_wave * (64 / e.group) > 64 * 1024) continue;
         out.push_back({e.group, e.wcap, L});
         if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU)) break;

@@ -27,6 +27,15 @@ enum ColStatus : int {
 constexpr int kWave = 64;
 constexpr int kNumWork = 8;
 
+/* Unrolled |J| width of the register-resident QR for a lane-group width. */
+__host__ __device__ constexpr int register_width(int group) { return group == 16 ? 8 : (group == 32 ? 16 : 32); }
+/* A size bin runs the register-resident QR when one group holds every row
+ * (|I| <= G) and |J| fits the unrolled width. */
+__host__ __device__ constexpr bool register_bin(int group, int capI, int capJ)
+{
+    return group > 0 && capI <= group && capJ <= register_width(group);
+}
+
 /* Device view of a CSC matrix (Compressed_Lines, src/Compressed_Lines.h:52-101). */
 struct DevCsc {
     int nrows, ncols;
@@ -47,6 +56,8 @@ struct ScratchLayout {
     int capU;      // distinct candidates |U|
     int capSort;   // raw index lists before sorting (a power of two)
     int capPairs;  // candidates on the std::sort tie path
+    int ldq;       // leading dimension of aq32: capI (FP32 copy of A(I,J)), or
+                   // capJ in register bins, where aq32 holds only the FP32 R
     int tie_in_main;        // pairs/stack in the main region (else in BuildArgs::tie_scratch)
     int max_rounds;         // most columns per group in one chunk of the work queue
     size_t tie_per_group;   // bytes of a group's tie region when not in the main one

@@ -46,7 +46,7 @@ struct WaveScratch {
     RhoPair* pairs;
     int* sel;
     int* stack;
-    float* aq32;  // mixed path: FP32 copy of A(I,J) (aqr keeps the FP64 one)
+    float* aq32;  // mixed path: FP32 copy of A(I,J) for the LDS QR, or the register QR's FP32 R
     float* bq32;  // mixed path: FP32 right-hand side
     double* g;    // mixed path: A(I,J)^T r
     double* y;    // mixed path: triangular-solve work vector; backward: m
@@ -186,8 +186,8 @@ __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& 
 
 /* Create_Submatrix_AHat (src/Spai_Sub.imp:221-282): A(I,J) column-major with
  * leading dimension capI, straight into the QR working array (and an FP32
- * copy for the mixed path).  Uses the column segments build_I left for this
- * J. */
+ * copy for the mixed path's LDS QR; the register QR converts as it loads).
+ * Uses the column segments build_I left for this J. */
 template <int G, bool kMixed>
 __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int m, int n)
@@ -448,16 +448,16 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
-    const int ld = L.capI;
+    const int ld = L.capI, ldq = L.ldq;
     const float* q = s.aq32;
     const bool row = lane < m;
     double a[NV], rc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
-        rc[j] = (lane < n && j < lane) ? (double)q[lane * ld + j] : 0.0;  // R(j, lane)
+        rc[j] = (lane < n && j < lane) ? (double)q[lane * ldq + j] : 0.0;  // R(j, lane)
     }
-    const double dinv = lane < n ? 1.0 / (double)q[lane * ld + lane] : 0.0;
+    const double dinv = lane < n ? 1.0 / (double)q[lane * ldq + lane] : 0.0;
     const double b0 = row ? s.b0[lane] : 0.0;
     double x = lane < n ? s.b[lane] : 0.0;
     for (int it = 0; it < kRefineSteps; ++it) {
@@ -487,7 +487,7 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
                 if (lane == K)
                     g = dk;
                 else if (lane < K)
-                    g = __fma_rn(-dk, (double)q[K * ld + lane], g);
+                    g = __fma_rn(-dk, (double)q[K * ldq + lane], g);
             }
         }
         x += g;
@@ -650,8 +650,9 @@ __device__ __forceinline__ int qr_f32(const ScratchLayout& L, const WaveScratch&
 
 /* FP32 Householder QR of A(I,J) with row r in the registers of group lane r
  * (|I| <= G, |J| <= NMAX), the FP32 right-hand side in slot NMAX; x0 =
- * R^-1 Q^T b in FP32.  Leaves R (upper triangle, column-major) in s.aq32 for
- * the refinement and x0 in s.b.  kFallback when min |R_ii| <= kCondGuard
+ * R^-1 Q^T b in FP32.  Rows come from the FP64 A(I,J) in s.aqr; leaves R
+ * (upper triangle, column-major, leading dimension L.ldq) in s.aq32 for the
+ * refinement and x0 in s.b.  kFallback when min |R_ii| <= kCondGuard
  * max |R_ii| (summation order is free here: the FP64 refinement decides). */
 template <int G, int NMAX>
 __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScratch& s, int m, int n)
@@ -664,7 +665,7 @@ __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScr
     const bool row = lane < m;
     float a[NMAX + 1];
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? s.aq32[j * ld + lane] : 0.f;
+    for (int j = 0; j < NMAX; ++j) a[j] = (row && j < n) ? (float)s.aqr[j * ld + lane] : 0.f;
     a[NMAX] = row ? (float)s.b[lane] : 0.f;
     float dmax = 0.f, dmin = INFINITY;
     const auto plus = [](float x, float y) { return x + y; };
@@ -744,7 +745,7 @@ __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScr
     }
 #pragma unroll
     for (int K = 0; K < NMAX; ++K)
-        if (K < n && lane <= K) s.aq32[K * ld + lane] = a[K];
+        if (K < n && lane <= K) s.aq32[K * L.ldq + lane] = a[K];
     if (lane < n) s.b[lane] = (double)x;
     __syncthreads();
     return 0;
@@ -767,7 +768,8 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
          * no barrier except after publishing x for the next residual */
         const bool row = lane < m;
         double x = lane < n ? s.b[lane] : 0.0;
-        const double dinv = lane < n ? 1.0 / (double)q[lane * ld + lane] : 0.0;  // R's reciprocal diagonal
+        const int ldq = L.ldq;
+        const double dinv = lane < n ? 1.0 / (double)q[lane * ldq + lane] : 0.0;  // R's reciprocal diagonal
         for (int it = 0; it < kRefineSteps; ++it) {
             double r = row ? s.b0[lane] : 0.0;  // r = b0 - A x (FP64)
             for (int jj = 0; jj < n; ++jj)
@@ -783,7 +785,7 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
                 for (int t = 0; t < 4; ++t)
                     if (lane == jb + t) g = p[t];
             }
-            g = normal_solve_lanes<G>(q, ld, g, n, dinv);  // R^T R d = g, R from the FP32 factor
+            g = normal_solve_lanes<G>(q, ldq, g, n, dinv);  // R^T R d = g, R from the FP32 factor
             x += g;
             const double dmx = Gr::max(lane < n ? fabs(g) : 0.0);
             const double xmx = Gr::max(lane < n ? fabs(x) : 0.0);
@@ -793,7 +795,8 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
         }
         return kFallback;
     } else {
-        if (qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
+        /* register bins hold register columns only (their aq32 has room for R alone) */
+        if (NMAX > 0 || qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
         /* x0 in FP32, column-oriented back substitution */
         for (int K = n - 1; K >= 0; --K) {
             float xk = s.bq32[K] / q[K * ld + K];
@@ -1441,7 +1444,7 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
     int m = build_I<G>(a, L, s, n);
     if (m == kOverflow) return kMoveOn;
     if (m < n) return kDone;  // degenerate column: no gradient
-    gather_ahat<G, kMixed>(a, L, s, m, n);
+    gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n);
     make_rhs<G>(a, s, m, col);
     /* the register QRs leave A(I,J) intact in LDS: products from it */
     const bool regs = NMAX > 0 && m <= G && n <= NMAX;
@@ -1450,8 +1453,8 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
     if (kMixed) {
         if (regs) {
             if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kRedoFp64;
-        } else if (qr_f32<G>(L, s, m, n) == kFallback) {
-            return kRedoFp64;
+        } else if (NMAX > 0 || qr_f32<G>(L, s, m, n) == kFallback) {
+            return kRedoFp64;  // (register bins hold register columns only)
         }
     } else if (regs) {
         int st2;
@@ -1471,14 +1474,14 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
     if (kMixed) {
         for (int i = lane; i < n; i += G) s.z[i] = s.g[i];
         __syncthreads();
-        normal_solve<G>(s.aq32, ld, s.g, n);
+        normal_solve<G>(s.aq32, L.ldq, s.g, n);
         bool ok = false;
         for (int it = 0; it < kRefineSteps; ++it) {
             csc_apply<G>(a, L, s, m, n, s.g, nullptr, s.b0, true);  // u = C lambda
             csc_apply_t<G>(a, L, s, m, n, s.b0, s.b, true);          // C^T u
             for (int i = lane; i < n; i += G) s.b[i] = s.z[i] - s.b[i];
             __syncthreads();
-            normal_solve<G>(s.aq32, ld, s.b, n);
+            normal_solve<G>(s.aq32, L.ldq, s.b, n);
             double dmx = 0.0, lmx = 0.0;
             for (int i = lane; i < n; i += G) {
                 double l = s.g[i] + s.b[i];
@@ -1930,7 +1933,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
     for (int step = -1; step < a.ns; ++step) {
         int m = build_I<G>(a, L, s, n);
         if (m == kOverflow) return kMoveOn;
-        gather_ahat<G, kMixed>(a, L, s, m, n);
+        gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n);
         st |= make_rhs<G>(a, s, m, col);
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {
@@ -2137,7 +2140,7 @@ using ColumnKernel = void (*)(BuildArgs, ScratchLayout, const int*, const int*, 
  * |J| fits the unrolled width. */
 static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
 {
-    const bool regs = mode != 4 && L.capI <= group && L.capJ <= (group == 16 ? 8 : (group == 32 ? 16 : 32));
+    const bool regs = mode != 4 && register_bin(group, L.capI, L.capJ);
 #define MSPAI_PICK(MODE)                                                                            \
     if (group == 16) return &spai_column_kernel<16, true, MODE, 0>;                                 \
     if (group == 32) return &spai_column_kernel<32, true, MODE, 0>;                                 \

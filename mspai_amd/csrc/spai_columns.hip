@@ -438,8 +438,8 @@ __device__ __forceinline__ double reduce_scatter(double (&v)[NV])
 }
 
 /* CSNE refinement of the FP32 register QR's solution for a group of G lanes
- * and |J| <= NV (used for 16-lane groups; in 32-lane groups it measured
- * slower than the LDS-fed steps below): row i of A(I,J) in FP64 and column i
+ * and |J| <= NV (16-lane groups, and 32-lane groups with |J| <= 8; with
+ * |J| <= 16 in 32-lane groups it measured slower than the LDS-fed steps): row i of A(I,J) in FP64 and column i
  * of R on lane i stay in registers across the steps; r = b0 - A x per row, A^T r by one
  * reduce-scatter, R^T R d = A^T r with the reciprocal diagonal, x += d.
  * Stops when max|d| <= 1e-10 max|x|; kFallback after kRefineSteps. */
@@ -451,11 +451,14 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
     const int ld = L.capI, ldq = L.ldq;
     const float* q = s.aq32;
     const bool row = lane < m;
-    double a[NV], rc[NV];
+    /* column `lane` of R in registers for 16-lane groups; 32-lane groups read
+     * it from LDS (registers would cost a wave per SIMD) */
+    constexpr bool kRcRegs = G == 16;
+    double a[NV], rc[kRcRegs ? NV : 1];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
-        rc[j] = (lane < n && j < lane) ? (double)q[lane * ldq + j] : 0.0;  // R(j, lane)
+        if constexpr (kRcRegs) rc[j] = (lane < n && j < lane) ? (double)q[lane * ldq + j] : 0.0;  // R(j, lane)
     }
     const double dinv = lane < n ? 1.0 / (double)q[lane * ldq + lane] : 0.0;
     const double b0 = row ? s.b0[lane] : 0.0;
@@ -477,7 +480,7 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
                 if (lane == K)
                     g = yk;
                 else if (lane > K && lane < n)
-                    g = __fma_rn(-yk, rc[K], g);
+                    g = __fma_rn(-yk, kRcRegs ? rc[kRcRegs ? K : 0] : (double)q[lane * ldq + K], g);
             }
         }
 #pragma unroll
@@ -761,9 +764,13 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
     const float* q = s.aq32;
     if (NMAX > 0 && m <= G && n <= NMAX) {
         if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kFallback;
-        /* 16-lane groups: the register-resident refinement; wider groups keep
-         * the LDS-fed one below (more registers would cost a wave per SIMD) */
+        /* 16-lane groups, and 32-lane groups with |J| <= 8: the
+         * register-resident refinement; wider columns keep the LDS-fed one
+         * below (their rows in registers would cost a wave per SIMD) */
         if constexpr (G == 16 && NMAX == 8) return refine_regs<G, 8>(L, s, m, n);
+        if constexpr (G == 32 && NMAX == 16) {
+            if (n <= 8) return refine_regs<G, 8>(L, s, m, n);
+        }
         /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
          * no barrier except after publishing x for the next residual */
         const bool row = lane < m;

@@ -64,13 +64,17 @@ __global__ void work_sum_kernel(const double* work, int n, double* out)
 }
 
 
+/* M's CSC from the per-column outputs: colptr[1..ncols] holds the inclusive
+ * scan of the lengths; this kernel also sets colptr[0] = 0 and *nnz. */
 __global__ void scatter_columns_kernel(int ncols, int cap, const int* m_len, const int* m_idx,
-                                       const double* m_val, const int* colptr, int* rowind, double* val)
+                                       const double* m_val, int* colptr, int* rowind, double* val, int* nnz)
 {
     int col = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (col >= ncols) return;
-    int len = m_len[col], o = colptr[col];
+    int len = m_len[col], o = col == 0 ? 0 : colptr[col];
+    if (lane == 0 && col == 0) colptr[0] = 0;
+    if (lane == 0 && col == ncols - 1) *nnz = o + len;
     size_t src = (size_t)col * cap;
     for (int i = lane; i < len; i += 64) {
         rowind[o + i] = m_idx[src + i];
@@ -751,7 +755,8 @@ static int waves_per_cu(const Bin& b, int mode)
  * the first queues; a bin's overflow appends to the next queue of its chain;
  * the mixed chain ends in FP64 queue 0, which also takes the columns the
  * mixed guards decline.  Counters: FP64 [0..8) lengths, [8..16) heads,
- * [16..24) routed lengths, mixed [24..32) lengths, [32..40) heads. */
+ * [16..24) routed lengths, mixed [24..32) lengths, [32..40) heads, [40]
+ * blocks of the routing kernel done. */
 static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins, const ScratchLayout& G,
                      int gwaves, bool mixed, int fp64_mode, int n)
 {
@@ -784,10 +789,9 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     }
     a.tie_scratch = ctx->d_tie;
     if (mixed) {
-        launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->stream);
+        launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->d_counter + 40, ctx->d_counter + 16,
+                        ctx->stream);
         HIPCHK(hipGetLastError());
-        launch_copy_ints(mcounts, ctx->d_counter + 16, nb, ctx->stream);
-        launch_copy_ints(counts, ctx->d_counter + 16 + nb, 1, ctx->stream);
         a.fallback = fq;
         a.n_fallback = counts;
         for (int b = 0; b < nb; ++b) {
@@ -802,9 +806,9 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         }
         a.status_or = COL_FALLBACK;
     } else {
-        launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->stream);
+        launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->d_counter + 40,
+                        ctx->d_counter + 16, ctx->stream);
         HIPCHK(hipGetLastError());
-        launch_copy_ints(counts, ctx->d_counter + 16, nb + 1, ctx->stream);
     }
     a.fallback = nullptr;
     a.n_fallback = nullptr;
@@ -981,11 +985,10 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     size_t cub_bytes = ctx->cub_bytes;
     HIPCHK(hipcub::DeviceScan::InclusiveSum(ctx->d_cub, cub_bytes, ctx->d_m_len, ctx->d_Mptr + 1, n,
                                              ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_Mptr, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(scatter_columns_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, cap,
-                       ctx->d_m_len, ctx->d_m_idx, ctx->d_m_val, ctx->d_Mptr, ctx->d_Mind, ctx->d_Mval);
+                       ctx->d_m_len, ctx->d_m_idx, ctx->d_m_val, ctx->d_Mptr, ctx->d_Mind, ctx->d_Mval,
+                       ctx->d_counter + 63);
     HIPCHK(hipGetLastError());
-    launch_copy_ints(ctx->d_Mptr + n, ctx->d_counter + 63, 1, ctx->stream);
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if ((rc = fetch_counters(ctx))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -2102,7 +2102,8 @@ __device__ __forceinline__ int distinct_rows_upto(const BuildArgs& a, int p0, in
  * exceeds the bin's |I| and |J0| <= 8; a column that still does not fit,
  * e.g. after augmentation, moves on by the overflow path). */
 __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, BinEdges edges, int* queues,
-                                                       int* counts, int* gqueue, int* gcount, int n)
+                                                       int* counts, int* gqueue, int* gcount, int n, int* done,
+                                                       int* routed)
 {
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     int bin = nbins;  // generic
@@ -2129,9 +2130,34 @@ __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, B
             break;
         }
     }
-    for (int b = 0; b < nbins; ++b)
-        wave_append(col < n && bin == b, col, queues + (size_t)b * n, counts + b);
-    wave_append(col < n && bin == nbins, col, gqueue, gcount);
+    /* one append per queue and wave, the atomics of all queues issued
+     * together: lane b adds the wave's count of queue b */
+    const int lane = lane_id();
+    const bool live = col < n;
+    unsigned long long mine = 0ull, lanes_b = 0ull;
+    for (int b = 0; b <= nbins; ++b) {
+        const unsigned long long mb = __ballot(live && bin == b);
+        if (bin == b) mine = mb;
+        if (lane == b) lanes_b = mb;
+    }
+    int base = 0;
+    if (lane <= nbins && lanes_b != 0ull) base = atomicAdd(lane < nbins ? counts + lane : gcount, __popcll(lanes_b));
+    base = __shfl(base, bin);
+    if (live) {
+        int* out = bin < nbins ? queues + (size_t)bin * n : gqueue;
+        out[base + __popcll(mine & ((1ull << lane) - 1ull))] = col;
+    }
+    /* the last block to finish records the routed queue lengths */
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && (int)threadIdx.x <= nbins) {
+        __threadfence();
+        int* c = (int)threadIdx.x < nbins ? counts + threadIdx.x : gcount;
+        routed[threadIdx.x] = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ void copy_ints_kernel(const int* src, int* dst, int count)
@@ -2199,10 +2225,10 @@ void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mo
 }
 
 void launch_classify(const BuildArgs& a, int nbins, const BinEdges& edges, int* queues, int* counts,
-                     int* gqueue, int* gcount, int n, hipStream_t stream)
+                     int* gqueue, int* gcount, int n, int* done, int* routed, hipStream_t stream)
 {
     hipLaunchKernelGGL(classify_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, nbins, edges, queues,
-                       counts, gqueue, gcount, n);
+                       counts, gqueue, gcount, n, done, routed);
 }
 
 void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream)

@@ -24,9 +24,11 @@ void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mo
 int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode);
 
 /* Route columns to bin queues: queue b < nbins at queues + b*n with its
- * length in counts[b]; the rest to gqueue (length gcount). */
+ * length in counts[b]; the rest to gqueue (length gcount).  routed[0..nbins]
+ * receives the lengths once routing is complete (counts[0..nbins), then
+ * *gcount); *done must be 0 at launch. */
 void launch_classify(const BuildArgs& a, int nbins, const BinEdges& edges, int* queues, int* counts,
-                     int* gqueue, int* gcount, int n, hipStream_t stream);
+                     int* gqueue, int* gcount, int n, int* done, int* routed, hipStream_t stream);
 
 /* dst[0..count) = src[0..count) on the stream (count <= 64). */
 void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream);

@@ -134,6 +134,7 @@ struct mspai_ctx {
     double *d_acsr_val = nullptr, *d_mcsr_val = nullptr;
     int* d_csr_cursor = nullptr;
     bool a_csr_ok = false, m_csr_ok = false;
+    bool ajn_ok = false;  // d_ajn holds the column norms of the current values
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     /* pattern->R cache */
     int cache_slots = 0, cache_capI = 0, cache_capJ = 0;
@@ -379,7 +380,7 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     ctx->maxcolnnz = maxc;
     ctx->maxrownnz = maxr;
     ctx->nnzM = -1;
-    ctx->a_csr_ok = ctx->m_csr_ok = false;
+    ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     ctx->pattern_nnz = -1;
     ctx->tuned_table = -1;
     if (ctx->block_bs != 1) {
@@ -401,7 +402,7 @@ extern "C" int mspai_update_values(mspai_ctx* ctx, const double* val)
     HIPCHK(hipMemcpyAsync(ctx->d_val, val, ctx->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
-    ctx->a_csr_ok = ctx->m_csr_ok = false;
+    ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     if (ctx->block_bs != 1) return block_convert(ctx);  // Convert_Block_Matrix_Update
     return MSPAI_OK;
 }
@@ -946,7 +947,11 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
 
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     BuildArgs a = base_args(ctx);
-    hipLaunchKernelGGL(column_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, a.A, ctx->d_ajn);
+    if (!ctx->ajn_ok) {  // once per set of values
+        hipLaunchKernelGGL(column_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, a.A, ctx->d_ajn);
+        HIPCHK(hipGetLastError());
+        ctx->ajn_ok = true;
+    }
     a.p_ptr = ctx->d_p_ptr;
     a.p_ind = ctx->d_p_ind;
     a.epsilon = opts->epsilon;
@@ -1021,7 +1026,7 @@ extern "C" int mspai_update_values_device(mspai_ctx* ctx, const double* d_val)
     HIPCHK(hipMemcpyAsync(ctx->d_val, d_val, ctx->nnz * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
-    ctx->a_csr_ok = ctx->m_csr_ok = false;
+    ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     if (ctx->block_bs != 1) return block_convert(ctx);
     return MSPAI_OK;
 }

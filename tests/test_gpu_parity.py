@@ -227,6 +227,36 @@ def test_autotuned_bins_bitwise(oracle, gpu):
     assert_bitwise(oracle, prob, ref, M, info)
 
 
+def test_new_values_adaptive_rebuild(oracle, gpu):
+    """An adaptive build, new values on the same pattern of A, then an adaptive
+    build from the start pattern again: bit for bit the oracle's on the new
+    values.  The new values rescale the columns (A D): the scores
+    ||r||^2 - (r^T a_j)^2 / ||a_j||^2 do not change and neither does the
+    pattern of M (M becomes D^-1 M), but scores formed with the previous
+    column norms would be off by d_j^2 and select other indices."""
+    prob = gen.config2(64)
+    b = gpu.Builder(0)
+    b.upload(prob.A, prob.row_pattern())
+    b.set_pattern(prob.P)
+    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+    M1 = b.download()
+    rng = np.random.Generator(np.random.PCG64(12))
+    A2 = sp.csc_matrix(prob.A @ sp.diags(rng.uniform(0.25, 4.0, prob.n)))
+    A2.sort_indices()
+    assert np.array_equal(A2.indptr, prob.A.indptr) and np.array_equal(A2.indices, prob.A.indices)
+    b.update_values(A2.data)
+    b.set_pattern(prob.P)
+    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+    M2 = b.download()
+    info = b.column_info()
+    b.close()
+    prob2 = gen.Problem("new_values", A2, prob.P, None, epsilon=prob.epsilon, maxnew=prob.maxnew,
+                        max_impr_steps=prob.max_impr_steps)
+    ref = oracle.spai(prob2, threads=THREADS)
+    assert_bitwise(oracle, prob2, ref, M2, info)
+    assert np.array_equal(M2.indptr, M1.indptr) and np.array_equal(M2.indices, M1.indices)
+
+
 def test_reuse_pattern_static_resolve(oracle, gpu):
     """Repeated setups: an adaptive build, new values on the same pattern of A,
     then the static re-solve on the previous M's pattern (P_Memory) equals the

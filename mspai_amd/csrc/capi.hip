@@ -151,6 +151,7 @@ struct mspai_ctx {
     int last_binning = 1;
     long long nnzB = 0;
     int last_routed[8] = {0};
+    int mixed_fp64_last = -1;  // FP64 queue length of the last mixed build (-1: none since the inputs changed)
     int last_queued[8] = {0};
     /* block SPAI (-bs): 1 = scalar, > 1 constant blocks, 0 = variable blocks */
     int block_bs = 1;
@@ -380,6 +381,7 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     ctx->maxcolnnz = maxc;
     ctx->maxrownnz = maxr;
     ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
     ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     ctx->pattern_nnz = -1;
     ctx->tuned_table = -1;
@@ -402,6 +404,7 @@ extern "C" int mspai_update_values(mspai_ctx* ctx, const double* val)
     HIPCHK(hipMemcpyAsync(ctx->d_val, val, ctx->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
     ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     if (ctx->block_bs != 1) return block_convert(ctx);  // Convert_Block_Matrix_Update
     return MSPAI_OK;
@@ -750,6 +753,9 @@ static int waves_per_cu(const Bin& b, int mode)
     return w;
 }
 
+/* Most FP64 columns of a mixed build that the generic kernel takes alone. */
+constexpr int kDirectFp64 = 4096;
+
 /* Launch routing and the bin chain over the columns whose patterns are in
  * a.p_ptr/p_ind.  Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb =
  * generic), mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills
@@ -806,6 +812,19 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             HIPCHK(hipGetLastError());
         }
         a.status_or = COL_FALLBACK;
+        /* the columns for FP64 (fallbacks, and what no mixed bin holds) go
+         * to the generic kernel in one launch when the last mixed build had
+         * few of them: the FP64 bins would mostly launch on empty queues.
+         * Both routes give the same bits. */
+        if (fp64_mode == 0 && ctx->mixed_fp64_last >= 0 && ctx->mixed_fp64_last <= kDirectFp64) {
+            a.fallback = nullptr;
+            a.n_fallback = nullptr;
+            a.overflow = fq;  // never written: the generic layout holds every column
+            a.n_overflow = counts + 7;
+            launch_fp64_generic(a, G, fp64_mode, fq, counts, heads + nb, gwaves, ctx->stream);
+            HIPCHK(hipGetLastError());
+            return MSPAI_OK;
+        }
     } else {
         launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->d_counter + 40,
                         ctx->d_counter + 16, ctx->stream);
@@ -1003,6 +1022,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     ctx->pattern_nnz = last;
     ctx->m_csr_ok = false;
     ctx->last_mixed = mixed;
+    ctx->mixed_fp64_last = mixed ? ctx->h_counter[0] : -1;
     ctx->last_binning = opts->binning;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -1026,6 +1046,7 @@ extern "C" int mspai_update_values_device(mspai_ctx* ctx, const double* d_val)
     HIPCHK(hipMemcpyAsync(ctx->d_val, d_val, ctx->nnz * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
     ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     if (ctx->block_bs != 1) return block_convert(ctx);
     return MSPAI_OK;
@@ -1039,6 +1060,7 @@ extern "C" int mspai_update_target_device(mspai_ctx* ctx, const double* d_val)
     HIPCHK(hipMemcpyAsync(ctx->d_b_val, d_val, ctx->nnzB * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
     return MSPAI_OK;
 }
 
@@ -1455,6 +1477,7 @@ extern "C" int mspai_set_block_size(mspai_ctx* ctx, int bs)
             ctx->pattern_set = false;
         }
         ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
         return MSPAI_OK;
     }
     if (ctx->nrows != ctx->ncols)
@@ -1486,6 +1509,7 @@ extern "C" int mspai_set_block_size(mspai_ctx* ctx, int bs)
     dfree(ctx->d_p_ind);
     ctx->pattern_set = false;
     ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
     ctx->pattern_nnz = -1;
     ctx->m_csr_ok = false;
     return MSPAI_OK;

@@ -67,3 +67,30 @@ def test_mixed_full_c3(oracle, gpu):
     M, info, stats = gpu.spai(prob, precision=1)
     worst, fbfrac = check_mixed(oracle, prob, M, info)
     print(f"   worst column rel err {worst:.2e}, fallback fraction {fbfrac:.4f}", flush=True)
+
+
+def test_mixed_repeat_routes_fp64_columns_alike(oracle, gpu):
+    """A mixed build whose FP64 remainder (guard fallbacks) is small sends it
+    to the generic kernel in the next build instead of through the FP64 bins:
+    the second build equals the first bit for bit, and both meet the bar."""
+    prob = gen.config3(n=4096, bw=300)
+    b = gpu.Builder(0)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        kw = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                  use_mean=prob.use_mean, precision=1)
+        st1 = b.build(**kw)
+        M1, info1 = b.download(), b.column_info()
+        st2 = b.build(**kw)
+        M2, info2 = b.download(), b.column_info()
+    finally:
+        b.close()
+    print(f"[mixed repeat] n={prob.n} fallbacks {st1['n_fallback']} / {st2['n_fallback']}", flush=True)
+    assert 0 < st1["n_fallback"] <= 4096 and st2["n_fallback"] == st1["n_fallback"]
+    check_mixed(oracle, prob, M1, info1)
+    assert np.array_equal(M1.indptr, M2.indptr) and np.array_equal(M1.indices, M2.indices)
+    assert np.array_equal(M1.data.view(np.int64), M2.data.view(np.int64))
+    assert np.array_equal(info1["res_norm"].view(np.int64), info2["res_norm"].view(np.int64))
+    assert np.array_equal(info1["nsolve"], info2["nsolve"]) and np.array_equal(info1["status"], info2["status"])
+

@@ -616,10 +616,14 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.sid = take(capSort * 4);
     L.wsum = take(8 * 8);
     /* register bins convert A(I,J) to FP32 as they load it: aq32 holds R only */
-    L.ldq = register_bin(group, (int)capI, capJ) ? capJ : (int)capI;
+    const bool regbin = register_bin(group, (int)capI, capJ);
+    L.ldq = regbin ? capJ : (int)capI;
     if (needs & kNeedMixed) L.aq32 = take((size_t)L.ldq * capJ * 4);
     if (needs & kNeedBackward) L.rq = take((size_t)capJ * capJ * 8);
-    if (needs & (kNeedMixed | kNeedBackward)) {
+    /* the FP32 right-hand side of the LDS QR and the vectors of its
+     * refinement, and the adjoint's vectors (the register QR and its
+     * refinement keep theirs in registers) */
+    if ((needs & kNeedBackward) || ((needs & kNeedMixed) && !regbin)) {
         L.bq32 = take(capB * 4);
         L.g = take((size_t)capJ * 8);
         L.y = take((size_t)capJ * 8);

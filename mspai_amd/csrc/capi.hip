@@ -708,6 +708,7 @@ static const BinTable kTables[] = {
       {64, 40, 128, 512, 1024, 4}}},
     {"g16-g64/occ8", 2, {{16, 8, 16, 64, 128, 8}, {64, 40, 128, 512, 1024, 0}}},
     {"g32-g64-g64/occ8", 3, {{32, 16, 32, 128, 256, 8}, {64, 32, 64, 256, 512, 0}, {64, 40, 128, 512, 1024, 0}}},
+    {"g16-g64/occ10", 2, {{16, 8, 16, 64, 128, 10}, {64, 40, 128, 512, 1024, 0}}},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 

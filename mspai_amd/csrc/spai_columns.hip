@@ -768,7 +768,7 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
          * register-resident refinement; wider columns keep the LDS-fed one
          * below (their rows in registers would cost a wave per SIMD) */
         if constexpr (G == 16 && NMAX == 8) return refine_regs<G, 8>(L, s, m, n);
-        if constexpr (G == 32 && NMAX == 16) {
+        if constexpr (G == 32) {
             if (n <= 8) return refine_regs<G, 8>(L, s, m, n);
         }
         /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
@@ -2170,7 +2170,8 @@ using ColumnKernel = void (*)(BuildArgs, ScratchLayout, const int*, const int*, 
 
 /* The LDS column kernel for a bin: register-resident QR (FP64 or mixed
  * build) when a group holds every row of the bin (|I| <= G) and the bin's
- * |J| fits the unrolled width. */
+ * |J| fits the unrolled width; 32-lane bins whose |J| is at most 8 take the
+ * 8-wide unrolling (fewer registers, more resident waves). */
 static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
 {
     const bool regs = mode != 4 && register_bin(group, L.capI, L.capJ);
@@ -2181,19 +2182,27 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
     switch (mode) {
     case 0:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 0, 8> : &spai_column_kernel<16, true, 0, 0>;
-        if (group == 32) return regs ? &spai_column_kernel<32, true, 0, 16> : &spai_column_kernel<32, true, 0, 0>;
+        if (group == 32)
+            return regs ? (L.capJ <= 8 ? &spai_column_kernel<32, true, 0, 8> : &spai_column_kernel<32, true, 0, 16>)
+                        : &spai_column_kernel<32, true, 0, 0>;
         return regs ? &spai_column_kernel<64, true, 0, 32> : &spai_column_kernel<64, true, 0, 0>;
     case 1:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 1, 8> : &spai_column_kernel<16, true, 1, 0>;
-        if (group == 32) return regs ? &spai_column_kernel<32, true, 1, 16> : &spai_column_kernel<32, true, 1, 0>;
+        if (group == 32)
+            return regs ? (L.capJ <= 8 ? &spai_column_kernel<32, true, 1, 8> : &spai_column_kernel<32, true, 1, 16>)
+                        : &spai_column_kernel<32, true, 1, 0>;
         return regs ? &spai_column_kernel<64, true, 1, 32> : &spai_column_kernel<64, true, 1, 0>;
     case 2:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 2, 8> : &spai_column_kernel<16, true, 2, 0>;
-        if (group == 32) return regs ? &spai_column_kernel<32, true, 2, 16> : &spai_column_kernel<32, true, 2, 0>;
+        if (group == 32)
+            return regs ? (L.capJ <= 8 ? &spai_column_kernel<32, true, 2, 8> : &spai_column_kernel<32, true, 2, 16>)
+                        : &spai_column_kernel<32, true, 2, 0>;
         return regs ? &spai_column_kernel<64, true, 2, 32> : &spai_column_kernel<64, true, 2, 0>;
     case 3:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 3, 8> : &spai_column_kernel<16, true, 3, 0>;
-        if (group == 32) return regs ? &spai_column_kernel<32, true, 3, 16> : &spai_column_kernel<32, true, 3, 0>;
+        if (group == 32)
+            return regs ? (L.capJ <= 8 ? &spai_column_kernel<32, true, 3, 8> : &spai_column_kernel<32, true, 3, 16>)
+                        : &spai_column_kernel<32, true, 3, 0>;
         return regs ? &spai_column_kernel<64, true, 3, 32> : &spai_column_kernel<64, true, 3, 0>;
     default: MSPAI_PICK(4);
     }

@@ -229,15 +229,17 @@ def main():
         opts = dict(base_opts, precision=precision, binning=2)
         for _ in range(args.warmup):
             b.build(**opts)
-        builds, solves = [], []
+        builds, solves, firsts = [], [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             st = b.build(**opts)
             builds.append(st["build_ms"])
             solves.append(st["solve_ms"])
+            firsts.append(st["first_bin_ms"])
         wall = time.perf_counter() - t0
         return {"ms_per_step": 1000.0 * wall / args.steps, "dev_ms": float(np.mean(builds)),
-                "kern_ms": float(np.mean(solves)), "M": b.download(), "work": b.work_totals(),
+                "kern_ms": float(np.mean(solves)), "first_ms": float(np.mean(firsts)),
+                "M": b.download(), "work": b.work_totals(),
                 "bins": b.bin_counts(), "n_fallback": st["n_fallback"], "n_overflow": st["n_overflow"]}
 
     r64 = timed(0)
@@ -273,7 +275,13 @@ def main():
     M = main_run["M"]
     ms_per_step = main_run["ms_per_step"]
     work = main_run["work"]
-    kern_ms = main_run["kern_ms"]
+    # the dominant kernel: the first size bin's launch when every column was
+    # queued there (its own HIP events); else all column-kernel launches
+    queued = main_run["bins"]["queued"]
+    first_all = bool(queued) and queued[0] == n and not any(queued[1:])
+    kern_ms = main_run["first_ms"] if first_all else main_run["kern_ms"]
+    kern_name = (f"spai_column_kernel, first size bin (all {n} columns)" if first_all
+                 else "spai_column_kernel (all size bins)")
     alg_bytes = _algorithmic_bytes(work, M.nnz, n)
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     traffic = _traffic(f"{args.workload}_{nx}")
@@ -363,7 +371,7 @@ def main():
         "fro_AM_minus_I": fro_gpu,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "spai_column_kernel (all size bins)", "kernel_ms": kern_ms,
+                     "kernel": kern_name, "kernel_ms": kern_ms, "all_bins_kernel_ms": main_run["kern_ms"],
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "qr_flop_per_launch": float(work[0] + work[1])},
         "bins": main_run["bins"],

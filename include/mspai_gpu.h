@@ -68,6 +68,8 @@ typedef struct {
     int n_fallback;         /* mixed path: columns recomputed in FP64 */
     int n_overflow;         /* columns that outgrew their size bin and moved on */
     int n_cache_hits;       /* least-squares solves served by the pattern->R cache */
+    double first_bin_ms;    /* device time of the first column-kernel launch (the first size bin,
+                               where routing sends every column that fits it) */
 } mspai_stats;
 
 /* Context on HIP device `device` (replaces the MPI world of Com_Server). */

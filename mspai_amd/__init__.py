@@ -38,7 +38,7 @@ class Opts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("solve_ms", C.c_double), ("nnz", C.c_longlong),
                 ("ncols", C.c_int), ("n_fallback", C.c_int), ("n_overflow", C.c_int),
-                ("n_cache_hits", C.c_int)]
+                ("n_cache_hits", C.c_int), ("first_bin_ms", C.c_double)]
 
 
 class TuneReport(C.Structure):
@@ -174,7 +174,7 @@ class Builder:
         self._chk(self.lib.mspai_build(self.h, C.byref(o), C.byref(st)), "mspai_build")
         return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz,
                 "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow,
-                "n_cache_hits": st.n_cache_hits}
+                "n_cache_hits": st.n_cache_hits, "first_bin_ms": st.first_bin_ms}
 
     # ---- device-pointer entry points (addresses of device buffers, e.g. tensor storage)
     def update_values_device(self, ptr: int):

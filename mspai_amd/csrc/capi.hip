@@ -135,7 +135,7 @@ struct mspai_ctx {
     int* d_csr_cursor = nullptr;
     bool a_csr_ok = false, m_csr_ok = false;
     bool ajn_ok = false;  // d_ajn holds the column norms of the current values
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // [4..5]: first size bin
     /* pattern->R cache */
     int cache_slots = 0, cache_capI = 0, cache_capJ = 0;
     int* d_cache_state = nullptr;
@@ -812,9 +812,11 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
             int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
             const ScratchLayout& L = bins[b].L;
+            if (b == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
                           ctx->num_cu * per_cu, ctx->stream);
             HIPCHK(hipGetLastError());
+            if (b == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
         }
         a.status_or = COL_FALLBACK;
         /* the columns for FP64 (fallbacks, and what no mixed bin holds) go
@@ -843,14 +845,19 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         a.n_overflow = counts + b + 1;
         int per_cu = waves_per_cu(bins[b], fp64_mode);
         const ScratchLayout& L = bins[b].L;
+        const bool first = b == 0 && !mixed;
+        if (first) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());
+        if (first) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     }
     a.overflow = fq;  // never written: the generic layout holds every column
     a.n_overflow = counts + 7;
+    if (nb == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));  // no bins: the generic kernel is the first
     launch_fp64_generic(a, G, fp64_mode, fq + (size_t)nb * n, counts + nb, heads + nb, gwaves, ctx->stream);
     HIPCHK(hipGetLastError());
+    if (nb == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     return MSPAI_OK;
 }
 
@@ -1036,6 +1043,9 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         HIPCHK(hipEventElapsedTime(&t_solve, ctx->ev[1], ctx->ev[2]));
         stats->build_ms = t_all;
         stats->solve_ms = t_solve;
+        float t_first = 0;
+        HIPCHK(hipEventElapsedTime(&t_first, ctx->ev[4], ctx->ev[5]));
+        stats->first_bin_ms = t_first;
         stats->nnz = last;
         stats->ncols = n;
         if ((rc = read_chain_counts(ctx, (int)bins.size(), mixed, stats))) return rc;
@@ -1121,6 +1131,9 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
         std::memset(stats, 0, sizeof(*stats));
         float t_all = 0, t_solve = 0;
         HIPCHK(hipEventElapsedTime(&t_all, ctx->ev[0], ctx->ev[3]));
+        float t_first = 0;
+        HIPCHK(hipEventElapsedTime(&t_first, ctx->ev[4], ctx->ev[5]));
+        stats->first_bin_ms = t_first;
         HIPCHK(hipEventElapsedTime(&t_solve, ctx->ev[1], ctx->ev[2]));
         stats->build_ms = t_all;
         stats->solve_ms = t_solve;
@@ -1673,6 +1686,7 @@ static int build_block(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stat
         HIPCHK(hipEventElapsedTime(&t_solve, ctx->ev[1], ctx->ev[2]));
         stats->build_ms = t_all;
         stats->solve_ms = t_solve;
+        stats->first_bin_ms = t_solve;  // the block column kernel
         stats->nnz = last;
         stats->ncols = nb;
     }

@@ -468,10 +468,18 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
 #pragma unroll
         for (int j = 0; j < NV; ++j)
             if (j < n) r = __fma_rn(-bcast_k<G>(x, j), a[j], r);
-        double v[NV];
+        /* (A^T r)_j on lane j: reduce-scatters of at most 16 sums (fewer
+         * live registers than one of NV) */
+        constexpr int CH = NV < 16 ? NV : 16;
+        double g = 0.0;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) v[j] = a[j] * r;
-        double g = __shfl(reduce_scatter<G, NV>(v), lane * (G / NV), G);  // (A^T r)_j on lane j
+        for (int c0 = 0; c0 < NV; c0 += CH) {
+            double v[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) v[j] = a[c0 + j] * r;
+            const double t = __shfl(reduce_scatter<G, CH>(v), (lane - c0) * (G / CH), G);
+            if (lane >= c0 && lane < c0 + CH) g = t;
+        }
         if (lane >= n) g = 0.0;
 #pragma unroll
         for (int K = 0; K < NV; ++K) {  // R^T y = g
@@ -771,6 +779,7 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
         if constexpr (G == 32) {
             if (n <= 8) return refine_regs<G, 8>(L, s, m, n);
         }
+        if constexpr (G == 64 && NMAX == 32) return refine_regs<G, 32>(L, s, m, n);
         /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
          * no barrier except after publishing x for the next residual */
         const bool row = lane < m;

@@ -437,6 +437,26 @@ __device__ __forceinline__ double reduce_scatter(double (&v)[NV])
     return v[0];
 }
 
+/* (A^T r)_j on lane j (0 on lanes >= n), with row i of A in a[] and r_i on
+ * lane i: reduce-scatters of at most 16 sums (fewer live registers than
+ * one of NV).  Mixed path only (not the 64-lane order). */
+template <int G, int NV>
+__device__ __forceinline__ double rows_t_times(const double (&a)[NV], double r, int n)
+{
+    constexpr int CH = NV < 16 ? NV : 16;
+    const int lane = Group<G>::lane();
+    double g = 0.0;
+#pragma unroll
+    for (int c0 = 0; c0 < NV; c0 += CH) {
+        double v[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) v[j] = a[c0 + j] * r;
+        const double t = __shfl(reduce_scatter<G, CH>(v), (lane - c0) * (G / CH), G);
+        if (lane >= c0 && lane < c0 + CH) g = t;
+    }
+    return lane < n ? g : 0.0;
+}
+
 /* CSNE refinement of the FP32 register QR's solution for a group of G lanes
  * and |J| <= NV (16-lane groups, and 32-lane groups with |J| <= 8; with
  * |J| <= 16 in 32-lane groups it measured slower than the LDS-fed steps): row i of A(I,J) in FP64 and column i
@@ -468,19 +488,7 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
 #pragma unroll
         for (int j = 0; j < NV; ++j)
             if (j < n) r = __fma_rn(-bcast_k<G>(x, j), a[j], r);
-        /* (A^T r)_j on lane j: reduce-scatters of at most 16 sums (fewer
-         * live registers than one of NV) */
-        constexpr int CH = NV < 16 ? NV : 16;
-        double g = 0.0;
-#pragma unroll
-        for (int c0 = 0; c0 < NV; c0 += CH) {
-            double v[CH];
-#pragma unroll
-            for (int j = 0; j < CH; ++j) v[j] = a[c0 + j] * r;
-            const double t = __shfl(reduce_scatter<G, CH>(v), (lane - c0) * (G / CH), G);
-            if (lane >= c0 && lane < c0 + CH) g = t;
-        }
-        if (lane >= n) g = 0.0;
+        double g = rows_t_times<G, NV>(a, r, n);  // (A^T r)_j on lane j
 #pragma unroll
         for (int K = 0; K < NV; ++K) {  // R^T y = g
             if (K < n) {
@@ -512,6 +520,44 @@ __device__ __forceinline__ int refine_regs(const ScratchLayout& L, const WaveScr
     }
     if (lane < n) s.b[lane] = x;
     __syncthreads();
+    return kFallback;
+}
+
+/* The mixed adjoint's solve R^T R lambda = gbar with the FP32 register QR's
+ * R, refined in FP64 as the forward solve is: rows of A(I,J) in registers,
+ * u = A lambda per row, A^T u by reduce-scatter, lambda += (R^T R)^-1
+ * (gbar - A^T u) until max|d| <= 1e-10 max|lambda|.  gbar in s.g on entry,
+ * lambda there on return; kFallback when it does not converge. */
+template <int G, int NV>
+__device__ __forceinline__ int refine_adjoint_regs(const ScratchLayout& L, const WaveScratch& s, int m, int n)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const int ld = L.capI, ldq = L.ldq;
+    const float* q = s.aq32;
+    const bool row = lane < m;
+    double a[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) a[j] = (row && j < n) ? s.aqr[j * ld + lane] : 0.0;
+    const double dinv = lane < n ? 1.0 / (double)q[lane * ldq + lane] : 0.0;
+    const double gbar = lane < n ? s.g[lane] : 0.0;
+    double lam = normal_solve_lanes<G>(q, ldq, gbar, n, dinv);
+    for (int it = 0; it < kRefineSteps; ++it) {
+        double u = 0.0;  // (A lambda)_i on lane i
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            if (j < n) u = __fma_rn(bcast_k<G>(lam, j), a[j], u);
+        const double h = rows_t_times<G, NV>(a, row ? u : 0.0, n);
+        const double d = normal_solve_lanes<G>(q, ldq, lane < n ? gbar - h : 0.0, n, dinv);
+        lam += d;
+        const double dmx = Gr::max(lane < n ? fabs(d) : 0.0);
+        const double lmx = Gr::max(lane < n ? fabs(lam) : 0.0);
+        if (dmx <= 1e-10 * lmx) {
+            if (lane < n) s.g[lane] = lam;
+            __syncthreads();
+            return 0;
+        }
+    }
     return kFallback;
 }
 
@@ -1487,7 +1533,13 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
     const bool dense = kMixed || regs;
     /* r = b - C m (into s.r) */
     csc_apply<G>(a, L, s, m, n, s.y, s.b0, s.r, dense);
-    if (kMixed) {
+    int rst = 1;  // 1: not refined in registers
+    if constexpr (kMixed && NMAX > 0) {
+        /* 16- and 64-lane groups, and 32-lane groups with |J| <= 8 */
+        if (regs && (G != 32 || n <= 8)) rst = refine_adjoint_regs<G, (G == 64 ? 32 : 8)>(L, s, m, n);
+        if (rst == kFallback) return kRedoFp64;
+    }
+    if (kMixed && rst != 0) {
         for (int i = lane; i < n; i += G) s.z[i] = s.g[i];
         __syncthreads();
         normal_solve<G>(s.aq32, L.ldq, s.g, n);
@@ -1514,7 +1566,7 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
             }
         }
         if (!ok) return kRedoFp64;
-    } else {
+    } else if (!kMixed) {
         normal_solve<G>(R, ldr, s.g, n);
     }
     /* s = C lambda (into s.b0) */

@@ -498,14 +498,21 @@ struct Group {
         return allreduce(p, [](int x, int y) { return x + y; });
     }
     /* 32-bit value of group lane `src` (a compile-time lane): readlane for
-     * a whole wave, ds_swizzle's broadcast mode inside 16/32-lane groups */
+     * a whole wave, DPP row_newbcast (lane SRC of each row of 16, a VALU
+     * move) for 16-lane groups, ds_swizzle's broadcast mode for 32-lane
+     * groups */
     template <int SRC>
     static __device__ __forceinline__ int lane_bcast(int v)
     {
         static_assert(SRC >= 0 && SRC < G, "source lane");
-        if (G == kWave) return __builtin_amdgcn_readlane(v, SRC);
-        /* lane l reads ((l & and) | or) within its 32-lane half */
-        return __builtin_amdgcn_ds_swizzle(v, (G == 16 ? 0x10 : 0x00) | (SRC << 5));
+        if constexpr (G == kWave) {
+            return __builtin_amdgcn_readlane(v, SRC);
+        } else if constexpr (G == 16) {
+            return __builtin_amdgcn_mov_dpp(v, 0x150 + SRC, 0xF, 0xF, false);
+        } else {
+            /* lane l reads ((l & and) | or) within its 32-lane half */
+            return __builtin_amdgcn_ds_swizzle(v, SRC << 5);
+        }
     }
     template <int SRC>
     static __device__ __forceinline__ double lane_bcast(double v)

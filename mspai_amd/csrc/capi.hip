@@ -1602,7 +1602,7 @@ static int build_block(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stat
     L.x = take((size_t)L.capJs * mb * sizeof(double));
     L.nbuf = take((size_t)kWave * 2 * mb * mb * sizeof(double));
     L.per_wave = off;
-    long long nwaves = std::min<long long>(nb, (long long)ctx->num_cu * 12);  // 3 per SIMD (the kernel's register budget)
+    long long nwaves = std::min<long long>(nb, (long long)ctx->num_cu * 16);  // 4 per SIMD (the kernel's register budget)
     nwaves = std::min<long long>(nwaves, (long long)(kBlockScratchBudget / L.per_wave));
     if (nwaves < 1) return fail(ctx, MSPAI_ERR_CAPACITY, "block build: working set of one column exceeds the budget");
     const size_t sbytes = (size_t)nwaves * L.per_wave;

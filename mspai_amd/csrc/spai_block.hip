@@ -535,7 +535,7 @@ __device__ bool block_column(const BlockArgs& a, const BlockLayout& L, const Blk
     return true;
 }
 
-__global__ __launch_bounds__(64, 3) void block_column_kernel(BlockArgs a, BlockLayout L)
+__global__ __launch_bounds__(64, 4) void block_column_kernel(BlockArgs a, BlockLayout L)
 {
     BlkScratch s = blk_scratch(a.scratch + (size_t)blockIdx.x * L.per_wave, L);
     for (;;) {

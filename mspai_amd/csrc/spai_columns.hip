@@ -2059,7 +2059,7 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
 /* kMode: 0 FP64 build, 1 mixed build, 2 FP64 adjoint, 3 mixed adjoint,
  * 4 FP64 build with the dense QR update (-qr 2). */
 template <int G, bool kLds, int kMode, int NMAX>
-__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? 4 : ((G == 64 && NMAX == 32 && kMode == 0) ? 3 : 1)) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
+__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? 4 : ((G == 64 && NMAX == 32 && kMode == 0) ? 3 : (!kLds ? 4 : 1))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
 {
     extern __shared__ __align__(16) char smem[];

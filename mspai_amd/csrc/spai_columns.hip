@@ -2063,13 +2063,19 @@ __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? 4 : ((G == 64 && NMAX 
                                                          const int* nqueue, int* head)
 {
     extern __shared__ __align__(16) char smem[];
+    /* an empty queue ends the wave before anything else: the working-set
+     * pointers below are live across the whole kernel, and what the register
+     * budget spills of them would otherwise be stored by every wave.  The
+     * length does not change during the launch (waves append to other
+     * queues only). */
+    const int nq = *nqueue;
+    if (nq <= 0) return;
     constexpr int NG = kWave / G;
     const int g = Group<G>::index();
     char* base = kLds ? smem + (size_t)g * L.per_wave
                       : a.scratch + ((size_t)blockIdx.x * NG + g) * L.per_wave;
     char* tie = L.tie_in_main ? base : a.tie_scratch + ((size_t)blockIdx.x * NG + g) * L.tie_per_group;
     WaveScratch s = wave_scratch(base, tie, L);
-    const int nq = *nqueue;
     /* columns per chunk: one per group, `rounds` rounds.  Every wave starts on
      * the chunk of its block index; when the queue fits in one chunk per wave
      * (at most L.max_rounds columns per group) that static split is all, else

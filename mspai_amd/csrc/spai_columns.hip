@@ -458,8 +458,9 @@ __device__ __forceinline__ double rows_t_times(const double (&a)[NV], double r, 
 }
 
 /* CSNE refinement of the FP32 register QR's solution for a group of G lanes
- * and |J| <= NV (16-lane groups, and 32-lane groups with |J| <= 8; with
- * |J| <= 16 in 32-lane groups it measured slower than the LDS-fed steps): row i of A(I,J) in FP64 and column i
+ * and |J| <= NV (16- and 64-lane groups, and 32-lane groups with |J| <= 8;
+ * with |J| <= 16 in 32-lane groups it measured slower than the LDS-fed
+ * steps): row i of A(I,J) in FP64 and column i
  * of R on lane i stay in registers across the steps; r = b0 - A x per row, A^T r by one
  * reduce-scatter, R^T R d = A^T r with the reciprocal diagonal, x += d.
  * Stops when max|d| <= 1e-10 max|x|; kFallback after kRefineSteps. */
@@ -818,9 +819,11 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
     const float* q = s.aq32;
     if (NMAX > 0 && m <= G && n <= NMAX) {
         if (qr_regs_f32<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n) == kFallback) return kFallback;
-        /* 16-lane groups, and 32-lane groups with |J| <= 8: the
-         * register-resident refinement; wider columns keep the LDS-fed one
-         * below (their rows in registers would cost a wave per SIMD) */
+        /* the register-resident refinement in 16- and 64-lane groups (the
+         * 64-lane bins are held to 8 waves per CU by LDS, so the registers
+         * cost no wave there) and in 32-lane groups with |J| <= 8; wider
+         * 32-lane columns keep the LDS-fed steps below (their rows in
+         * registers would cost a wave per SIMD) */
         if constexpr (G == 16 && NMAX == 8) return refine_regs<G, 8>(L, s, m, n);
         if constexpr (G == 32) {
             if (n <= 8) return refine_regs<G, 8>(L, s, m, n);
@@ -2057,7 +2060,11 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
 }
 
 /* kMode: 0 FP64 build, 1 mixed build, 2 FP64 adjoint, 3 mixed adjoint,
- * 4 FP64 build with the dense QR update (-qr 2). */
+ * 4 FP64 build with the dense QR update (-qr 2).  The second launch bound is
+ * the resident waves per SIMD the register allocation must allow where LDS
+ * would admit more than the registers otherwise do: 4 for the 8-wide
+ * 32-lane kernels and the generic (HBM working set) kernels, 3 for the FP64
+ * 64-lane register kernel (each measured faster despite a small spill). */
 template <int G, bool kLds, int kMode, int NMAX>
 __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? 4 : ((G == 64 && NMAX == 32 && kMode == 0) ? 3 : (!kLds ? 4 : 1))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)

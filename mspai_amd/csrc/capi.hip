@@ -925,7 +925,7 @@ static int ensure_cache(mspai_ctx* ctx, int entries, int capI, int capJ)
 /* Generic-kernel sizing: keep its HBM working sets within 16 GiB. */
 static int prepare_generic(mspai_ctx* ctx, const ScratchLayout& G, int* gwaves)
 {
-    int w = std::max(1, ctx->num_cu) * 16;
+    int w = std::max(1, ctx->num_cu) * 16;  // 4 waves per SIMD, the generic kernels' register budget
     const size_t budget = (size_t)16 << 30;
     while (w > 1 && (size_t)w * G.per_wave > budget) w /= 2;
     if ((size_t)w * G.per_wave > budget) return fail(ctx, MSPAI_ERR_CAPACITY, "per-column working set too large");

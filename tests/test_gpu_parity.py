@@ -227,6 +227,18 @@ def test_autotuned_bins_bitwise(oracle, gpu):
     assert_bitwise(oracle, prob, ref, M, info)
 
 
+def test_build_stats_timings(gpu):
+    """The build's event timings nest: the first size bin's launch lies inside
+    the column-kernel window, which lies inside the whole build (FP64 and
+    mixed builds, binned and generic-only)."""
+    prob = gen.config2(64)
+    for opts in (dict(precision=0, binning=1), dict(precision=1, binning=1), dict(precision=0, binning=0)):
+        M, info, st = gpu.spai(prob, **opts)
+        print(f"[stats {opts}] build {st['build_ms']:.4f} solve {st['solve_ms']:.4f} "
+              f"first {st['first_bin_ms']:.4f} ms", flush=True)
+        assert 0.0 < st["first_bin_ms"] <= st["solve_ms"] <= st["build_ms"]
+
+
 def test_new_values_adaptive_rebuild(oracle, gpu):
     """An adaptive build, new values on the same pattern of A, then an adaptive
     build from the start pattern again: bit for bit the oracle's on the new

@@ -2062,11 +2062,12 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
 /* kMode: 0 FP64 build, 1 mixed build, 2 FP64 adjoint, 3 mixed adjoint,
  * 4 FP64 build with the dense QR update (-qr 2).  The second launch bound is
  * the resident waves per SIMD the register allocation must allow where LDS
- * would admit more than the registers otherwise do: 4 for the 8-wide
- * 32-lane kernels and the generic (HBM working set) kernels, 3 for the FP64
- * 64-lane register kernel (each measured faster despite a small spill). */
+ * would admit more than the registers otherwise do: 5 for the FP64 8-wide
+ * 32-lane kernel, 4 for its other modes and the generic (HBM working set)
+ * kernels, 3 for the FP64 64-lane register kernel (each measured faster
+ * despite the spill; the mixed 8-wide kernel is slower at 5). */
 template <int G, bool kLds, int kMode, int NMAX>
-__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? 4 : ((G == 64 && NMAX == 32 && kMode == 0) ? 3 : (!kLds ? 4 : 1))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
+__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4) : ((G == 64 && NMAX == 32 && kMode == 0) ? 3 : (!kLds ? 4 : 1))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
 {
     extern __shared__ __align__(16) char smem[];

@@ -183,8 +183,8 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
         return
     if args.gpus != 1:
-        print("bench.py builds on one GPU only: the column solve shards across GPUs by "
-              "columns (see DESIGN.md), but multi-GPU runs are not part of this build.",
+        print("bench.py runs on one GPU only: the columns are independent and would be split "
+              "across GPUs by column ranges, but only single-GPU runs are supported.",
               file=sys.stderr)
         sys.exit(2)
 

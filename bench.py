@@ -1,16 +1,21 @@
 """Benchmark: SPAI columns/s on one MI355X.
 
-Workload (the configuration the metric is quoted on): adaptive SPAI on the 2D
-upwind convection-diffusion operator, 256 x 256 grid (n = 65,536), eps 0.4,
-5 augmentation steps, 5 new indices per step, start pattern diag(A).  One
-step = one complete build of M (every column of M) on a device-resident
-copy of A; inputs are uploaded before the timed region.
+Workload (the configuration the metric is quoted on): adaptive SPAI on the
+random nonsymmetric band matrix, n = 262,144 (per row the diagonal plus 11
+off-diagonal columns uniform within +-2000, values N(0,1), diagonal 1.2 x the
+row's absolute sum; seed 20251202), eps 0.3, 8 augmentation steps, 4 new
+indices per step, start pattern diag(A).  One step = one complete build of M
+(every column) on a device-resident copy of A; inputs are uploaded before
+the timed region.
 
 Prints one JSON line.  `value` is the columns/s of the mixed-precision build
-(FP32 QR + FP64 refinement); the FP64 build is timed beside it.
-`--workload` selects another of the five workloads for profiling runs.
-`roofline` refers to the column kernel (HIP events on the library's stream);
+(FP32 QR + FP64 refinement); the FP64 build is timed beside it, and so are
+the size-binned against the unbinned (generic kernel only) FP64 build, the
+MSPAI probing workload (3D Laplacian 64^3, rho = 100) with its adjoint, and
+the block SPAI build.  `roofline` refers to the dominant column-kernel launch
+of the mixed build (HIP events on the library's stream around every launch);
 `cpu_baseline` times the CPU FP64 restatement of the reference on this host.
+`--workload` selects another workload as the main line (profiling runs).
 """
 from __future__ import annotations
 
@@ -38,21 +43,18 @@ WORKLOADS = {
 }
 
 
-def _algorithmic_bytes(work, nnzM, n):
-    """Gather + candidate union + rho reads + the CSC output of M."""
-    return float(work[2] + work[3] + work[4] + 12.0 * nnzM + 4.0 * (n + 1))
-
-
 def _traffic(name):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/pmc_<workload>.json),
+    with the kernel it was taken on, or (None, None)."""
     path = os.path.join(ROOT, "profiles", f"pmc_{name}.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         d = json.load(open(path))
-        return float(d["hbm_bytes_per_launch"])
+        return float(d["hbm_bytes_per_launch"]), d.get("kernel")
     except Exception:
-        return None
+        return None, None
 
 
 def _fro_residual(A, M, B=None):
@@ -137,7 +139,7 @@ def _block_bench(mspai_amd, gen, steps, warmup, with_cpu):
            "mean_solves_per_block_column": float(np.mean(info["nsolve"][:nb]))}
     if with_cpu:
         from oracle import block as OB
-        cores = min(os.cpu_count() or 1, 16)
+        cores, _ = _cpu_cores()
         Bm = OB.BlockMatrix(A, OB.constant_blocks(n, 3))
         t0 = time.perf_counter()
         reps = 0
@@ -157,15 +159,164 @@ def _block_bench(mspai_amd, gen, steps, warmup, with_cpu):
     return out
 
 
+def _cpu_cores():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    at most 16 (one GPU's share of the host on a shared GPU box)."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return min(visible, 16), visible
+
+
+def _problem(name, nx=None):
+    from mspai_amd import gen
+    make = getattr(gen, WORKLOADS[name][0])
+    if name == "random":
+        prob = make(n=nx) if nx else make()
+        size = prob.n
+    else:
+        prob = make(nx) if nx else make()
+        size = nx or (round(prob.n ** 0.5) if name in ("laplace2d", "convdiff2d") else round(prob.n ** (1 / 3)))
+    return prob, size
+
+
+def _builder(mspai_amd, prob):
+    b = mspai_amd.Builder(0)
+    b.upload(prob.A, prob.row_pattern())
+    b.set_pattern(prob.P)
+    if prob.B is not None:
+        b.set_target(prob.B)
+    return b
+
+
+def _timed(b, opts, steps, warmup):
+    """Wall time per build over `steps` builds after `warmup`, with the
+    per-launch device times of every timed build averaged."""
+    for _ in range(warmup):
+        b.build(**opts)
+    builds, launches = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = b.build(**opts)
+        builds.append(st["build_ms"])
+        launches.append(b.launch_stats())
+    wall = time.perf_counter() - t0
+    names = [x["name"] for x in launches[0]]
+    per_launch = [{"name": nm, "ms": float(np.mean([ls[k]["ms"] for ls in launches]))} for k, nm in enumerate(names)]
+    return {"ms_per_step": 1000.0 * wall / steps, "dev_ms": float(np.mean(builds)), "launches": per_launch,
+            "n_fallback": st["n_fallback"], "n_overflow": st["n_overflow"], "bins": b.bin_counts()}
+
+
+def _counted(b, opts):
+    """One build with the work counters on: per-launch columns, algorithmic
+    counts and near-margin decisions, M and the per-column info."""
+    b.build(**dict(opts, counters=1))
+    return {"launches": b.launch_stats(), "work": b.work_totals(), "M": b.download(), "info": b.column_info()}
+
+
+def _dominant(timed, counted):
+    """The launch with the largest device time, its algorithmic bytes
+    (gather + candidate union + rho reads of the columns it finished, plus
+    their CSC output 12 |J| + 4) and achieved GB/s."""
+    k = int(np.argmax([x["ms"] for x in timed["launches"]]))
+    c = counted["launches"][k]
+    alg = float(c["work"][2] + c["work"][3] + c["work"][4] + 12.0 * c["nnz"] + 4.0 * c["columns"])
+    ms = timed["launches"][k]["ms"]
+    return {"kernel": timed["launches"][k]["name"], "kernel_ms": ms, "columns": int(c["columns"]),
+            "algorithmic_bytes_per_launch": alg, "achieved": alg / (ms * 1e-3) / 1e9,
+            "qr_flop_per_launch": float(c["work"][0] + c["work"][1])}
+
+
+def _launch_table(timed, counted):
+    return [{"name": t["name"], "ms": t["ms"], "columns": int(c["columns"]), "nnz": int(c["nnz"]),
+             "near_margin": [int(v) for v in c["near"]]}
+            for t, c in zip(timed["launches"], counted["launches"])]
+
+
+def _probing_leg(mspai_amd, gen, torch, steps, warmup):
+    """MSPAI probing on the 3D Laplacian 64^3 (n = 262,144), static
+    pattern(A), rho = 100: FP64 and mixed builds on their autotuned bins,
+    and the adjoint's backward/forward time."""
+    prob = gen.config4()
+    b = _builder(mspai_amd, prob)
+    base = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=0, binning=2)
+    out = {"workload": "MSPAI probing, 3D 7-point Laplacian 64^3 (n=262144), rho=100, P=pattern(A)"}
+    for label, prec in (("fp64", 0), ("mixed", 1)):
+        tune = b.autotune(repeats=1, precision=prec, max_impr_steps=0)
+        t = _timed(b, dict(base, precision=prec), steps, warmup)
+        out[label] = {"columns_per_s": prob.n / (t["ms_per_step"] * 1e-3), "ms_per_build": t["ms_per_step"],
+                      "device_ms_per_build": t["dev_ms"], "bins": tune["name"], "n_fallback": t["n_fallback"]}
+        if label == "fp64":
+            M64 = b.download()
+    M = b.download()
+    same = bool(np.array_equal(M64.indptr, M.indptr) and np.array_equal(M64.indices, M.indices))
+    out["mixed_over_fp64"] = out["mixed"]["columns_per_s"] / out["fp64"]["columns_per_s"]
+    out["mixed_pattern_equal_fp64"] = same
+    out["mixed_max_column_rel_err_vs_fp64"] = _column_rel_err(M, M64) if same else None
+    b.close()
+    if torch is not None:
+        out["bwd_fwd"] = _bwd_fwd(mspai_amd, gen, torch, prob, steps=3)
+    return out
+
+
+def _cpu_leg(prob, M64, Mmx, sample):
+    """The oracle (FP64, -O3) on this host: `sample` columns (None = all)
+    on every usable core for about 10 s, plus its 1-thread rate; and the
+    parity of the GPU builds on those columns."""
+    from oracle import oracle as O
+    cores, visible = _cpu_cores()
+    n = prob.n
+    cols = None
+    if sample:
+        cols = np.sort(np.random.Generator(np.random.PCG64(3)).choice(n, size=sample, replace=False)).astype(np.int32)
+    ncols = cols.size if cols is not None else n
+    t = time.perf_counter()
+    reps = 0
+    while True:
+        ref = O.spai(prob, threads=cores, cols=cols)
+        reps += 1
+        if time.perf_counter() - t > 10.0 or reps >= 50:
+            break
+    cpu_t = (time.perf_counter() - t) / reps
+    one = cols[:1024] if cols is not None else None
+    t = time.perf_counter()
+    O.spai(prob, threads=1, cols=one)
+    cpu1 = (time.perf_counter() - t) / (one.size if one is not None else n)
+    idx = cols if cols is not None else np.arange(n)
+    pat = val = mpat = True
+    rel = 0.0
+    for tt, k in enumerate(idx):
+        ln = ref["len"][tt]
+        ri, rv = ref["idx"][tt, :ln], ref["val"][tt, :ln]
+        a0, a1 = M64.indptr[k], M64.indptr[k + 1]
+        pat &= bool(np.array_equal(M64.indices[a0:a1], ri))
+        val &= bool(np.array_equal(M64.data[a0:a1].view(np.int64), rv.view(np.int64)))
+        if Mmx is not None:
+            b0, b1 = Mmx.indptr[k], Mmx.indptr[k + 1]
+            same = bool(np.array_equal(Mmx.indices[b0:b1], ri))
+            mpat &= same
+            if same:
+                rel = max(rel, float(np.linalg.norm(Mmx.data[b0:b1] - rv) / max(np.linalg.norm(rv), 1e-300)))
+    what = f"stratified sample of {ncols} of {n} columns (seed 3)" if cols is not None else f"full workload (n={n})"
+    cpu = {"value": ncols / cpu_t, "unit": "columns/s", "cores": cores, "visible_cores": visible, "kind": "port",
+           "sample": f"{what} x {reps} runs on {cores} threads ({cpu_t:.2f} s per run); 1 thread: "
+                     f"{1.0 / cpu1:.0f} columns/s",
+           "single_thread_value": 1.0 / cpu1}
+    parity = {"columns": int(ncols), "fp64_pattern_equal": pat, "fp64_values_bitwise": val,
+              "near_margin_decisions_oracle": [int(v) for v in ref["near"].sum(axis=0)]}
+    if Mmx is not None:
+        parity.update({"mixed_pattern_equal": mpat, "mixed_max_column_rel_err_vs_oracle": rel})
+    return cpu, parity
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nx", type=int, default=None, help="grid size (default: the workload's full size)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["block"], default="convdiff2d",
+    ap.add_argument("--nx", type=int, default=None, help="grid size / n (default: the workload's full size)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["block"], default="random",
                     help="block: only the block SPAI build (-bs 3) on the coupled 3-field system")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="only the main workload (profiling runs)")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="timed paths; with both, value is the mixed path")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -205,146 +356,79 @@ def main():
                           "cpu_baseline": blk.get("cpu_baseline")}))
         return
 
-    make = getattr(gen, WORKLOADS[args.workload][0])
-    if args.workload == "random":
-        prob = make(n=args.nx) if args.nx else make()
-    else:
-        prob = make(args.nx) if args.nx else make()
+    prob, size = _problem(args.workload, args.nx)
     n = prob.n
-    if args.workload == "random":
-        nx = n
-    else:
-        nx = args.nx or (round(n ** 0.5) if args.workload in ("laplace2d", "convdiff2d") else round(n ** (1 / 3)))
-    b = mspai_amd.Builder(0)
-    b.upload(prob.A, prob.row_pattern())
-    b.set_pattern(prob.P)
-    if prob.B is not None:
-        b.set_target(prob.B)
-    base_opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
-                     use_mean=prob.use_mean)
-    # one-time set-up outside the timed region: choose the size bins
-    tune = b.autotune(repeats=2, **base_opts)
-
-    def timed(precision):
-        opts = dict(base_opts, precision=precision, binning=2)
-        for _ in range(args.warmup):
-            b.build(**opts)
-        builds, solves, firsts = [], [], []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = b.build(**opts)
-            builds.append(st["build_ms"])
-            solves.append(st["solve_ms"])
-            firsts.append(st["first_bin_ms"])
-        wall = time.perf_counter() - t0
-        return {"ms_per_step": 1000.0 * wall / args.steps, "dev_ms": float(np.mean(builds)),
-                "kern_ms": float(np.mean(solves)), "first_ms": float(np.mean(firsts)),
-                "M": b.download(), "work": b.work_totals(),
-                "bins": b.bin_counts(), "n_fallback": st["n_fallback"], "n_overflow": st["n_overflow"]}
-
-    r64 = timed(0)
-    tune_mixed = None
-    rmx = None
+    print(f"[bench] {args.workload} n={n} nnz(A)={prob.A.nnz}", file=sys.stderr, flush=True)
+    b = _builder(mspai_amd, prob)
+    base = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                use_mean=prob.use_mean)
+    # one-time set-up outside the timed regions: choose the size bins per precision
+    tune = b.autotune(repeats=2, **base)
+    r64 = _timed(b, dict(base, precision=0, binning=2), args.steps, args.warmup)
+    c64 = _counted(b, dict(base, precision=0, binning=2))
+    print(f"[bench] fp64 {r64['ms_per_step']:.3f} ms/build", file=sys.stderr, flush=True)
+    tune_mixed = rmx = cmx = None
     if args.precision in ("mixed", "both"):
-        # the mixed kernels have their own register and LDS budgets: tune them apart
-        tune_mixed = b.autotune(repeats=2, precision=1, **base_opts)
-        rmx = timed(1)
-    # the reference's LRU cache option (-cs): same bits, repeated A(I,J) blocks reuse factors
+        tune_mixed = b.autotune(repeats=2, precision=1, **base)
+        rmx = _timed(b, dict(base, precision=1, binning=2), args.steps, args.warmup)
+        cmx = _counted(b, dict(base, precision=1, binning=2))
+        print(f"[bench] mixed {rmx['ms_per_step']:.3f} ms/build", file=sys.stderr, flush=True)
+    unbinned = None
+    if not args.no_legs and args.workload == "random":
+        # the generic kernel alone (no size bins): the unbinned baseline
+        r0 = _timed(b, dict(base, precision=0, binning=0), max(2, args.steps // 3), 1)
+        unbinned = {"fp64_unbinned_ms_per_build": r0["ms_per_step"], "fp64_binned_ms_per_build": r64["ms_per_step"],
+                    "binned_speedup": r0["ms_per_step"] / r64["ms_per_step"]}
     cache_run = None
-    if args.precision == "both" and args.workload == "convdiff2d":
-        opts_c = dict(base_opts, precision=0, binning=2, cache=65536)
-        for _ in range(args.warmup):
-            b.build(**opts_c)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            stc = b.build(**opts_c)
-        cache_run = {"columns_per_s": n / ((time.perf_counter() - t0) / args.steps),
+    if not args.no_legs and args.workload in ("random", "convdiff2d"):
+        # the reference's LRU cache option (-cs): same bits, repeated A(I,J) blocks reuse factors
+        c2 = gen.config2() if args.workload == "random" else prob
+        bc = _builder(mspai_amd, c2)
+        o2 = dict(epsilon=c2.epsilon, maxnew=c2.maxnew, max_impr_steps=c2.max_impr_steps, precision=0, binning=1)
+        plain = _timed(bc, o2, args.steps, args.warmup)
+        Mp = bc.download()
+        cached = _timed(bc, dict(o2, cache=65536), args.steps, args.warmup)
+        stc = bc.build(**dict(o2, cache=65536))
+        cache_run = {"workload": WORKLOADS["convdiff2d"][1].format(nx=256, n=c2.n),
+                     "columns_per_s_without": c2.n / (plain["ms_per_step"] * 1e-3),
+                     "columns_per_s_with_cache": c2.n / (cached["ms_per_step"] * 1e-3),
                      "cache_hits_per_build": stc["n_cache_hits"],
-                     "pattern_equal_fp64": bool(np.array_equal(b.download().indices, r64["M"].indices))}
+                     "bitwise_equal": bool(np.array_equal(bc.download().data.view(np.int64), Mp.data.view(np.int64)))}
+        bc.close()
     b.close()
+    probing = None
+    if not args.no_legs and args.workload == "random" and args.precision == "both":
+        probing = _probing_leg(mspai_amd, gen, torch, args.steps, args.warmup)
     bwd = None
-    if torch is not None and args.precision == "both":
-        if args.workload == "convdiff2d":
-            bwd = _bwd_fwd(mspai_amd, gen, torch)
-        elif args.workload in ("varcoef3d", "probing3d", "laplace2d"):
-            bwd = _bwd_fwd(mspai_amd, gen, torch, prob, steps=3)
+    if not args.no_legs and torch is not None and args.precision == "both" and args.workload in (
+            "varcoef3d", "probing3d", "laplace2d"):
+        bwd = _bwd_fwd(mspai_amd, gen, torch, prob, steps=3)
     block = None
-    if args.precision == "both" and args.workload == "convdiff2d":
+    if not args.no_legs and args.precision == "both" and args.workload == "random":
         block = _block_bench(mspai_amd, gen, max(3, args.steps // 2), args.warmup, not args.no_cpu)
-    main_run = rmx if rmx is not None else r64
-    M = main_run["M"]
-    ms_per_step = main_run["ms_per_step"]
-    work = main_run["work"]
-    # the dominant kernel: the first size bin's launch when every column was
-    # queued there (its own HIP events); else all column-kernel launches
-    queued = main_run["bins"]["queued"]
-    first_all = bool(queued) and queued[0] == n and not any(queued[1:])
-    kern_ms = main_run["first_ms"] if first_all else main_run["kern_ms"]
-    kern_name = (f"spai_column_kernel, first size bin (all {n} columns)" if first_all
-                 else "spai_column_kernel (all size bins)")
-    alg_bytes = _algorithmic_bytes(work, M.nnz, n)
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    traffic = _traffic(f"{args.workload}_{nx}")
+
+    main_t, main_c = (rmx, cmx) if rmx is not None else (r64, c64)
+    M = main_c["M"]
+    M64 = c64["M"]
+    dom = _dominant(main_t, main_c)
+    traffic, traffic_kernel = _traffic(f"{args.workload}_{size}")
     fro_gpu = _fro_residual(prob.A, M, prob.B)
-    M64 = r64["M"]
     mixed_check = None
     if rmx is not None:
         same_pat = bool(np.array_equal(M64.indptr, M.indptr) and np.array_equal(M64.indices, M.indices))
-        rel = float(np.max(np.abs(M.data - M64.data) / np.maximum(np.abs(M64.data), 1e-300))) if same_pat else None
         fro64 = _fro_residual(prob.A, M64, prob.B)
         mixed_check = {"pattern_equal_fp64": same_pat,
                        "max_column_rel_err_vs_fp64": _column_rel_err(M, M64) if same_pat else None,
-                       "max_entry_rel_err_vs_fp64": rel,
-                       "fro_AM_minus_I_fp64": fro64,
-                       "fro_rel_diff": abs(fro_gpu - fro64) / fro64,
-                       "n_fp64_fallback": rmx["n_fallback"]}
-
-    cpu = None
-    parity = None
+                       "fro_AM_minus_I_fp64": fro64, "fro_rel_diff": abs(fro_gpu - fro64) / fro64,
+                       "n_fp64_fallback": rmx["n_fallback"],
+                       "fp64_fallback_fraction": rmx["n_fallback"] / n}
+    cpu = parity = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        cores = min(os.cpu_count() or 1, 16)
-        # the two large workloads: a stratified sample of 8,192 columns (seed 3)
-        sampled = args.workload in ("random", "varcoef3d")
-        cols = (np.sort(np.random.Generator(np.random.PCG64(3)).choice(n, size=8192, replace=False)).astype(np.int32)
-                if sampled else None)
-        ncols = cols.size if sampled else n
-        t = time.perf_counter()
-        reps = 0
-        while True:
-            ref = O.spai(prob, threads=cores, cols=cols)
-            reps += 1
-            if time.perf_counter() - t > 5.0 or reps >= 50:
-                break
-        cpu_t = (time.perf_counter() - t) / reps
-        one = cols[:1024] if sampled else None
-        t = time.perf_counter()
-        O.spai(prob, threads=1, cols=one)
-        cpu1 = (time.perf_counter() - t) / (one.size if sampled else n)
-        if sampled:
-            ok_pat = ok_val = True
-            for tt, k in enumerate(cols):
-                a0, a1 = M64.indptr[k], M64.indptr[k + 1]
-                ok_pat &= bool(np.array_equal(M64.indices[a0:a1], ref["idx"][tt, :ref["len"][tt]]))
-                ok_val &= bool(np.array_equal(M64.data[a0:a1].view(np.int64),
-                                              ref["val"][tt, :ref["len"][tt]].view(np.int64)))
-            parity = {"sampled_columns": int(cols.size), "fp64_pattern_equal": ok_pat, "fp64_values_bitwise": ok_val}
-        else:
-            Mref = O.to_csc(ref, n)
-            parity = {"fp64_pattern_equal": bool(np.array_equal(Mref.indptr, M64.indptr)
-                                                 and np.array_equal(Mref.indices, M64.indices)),
-                      "fp64_values_bitwise": bool(np.array_equal(Mref.data.view(np.int64),
-                                                                 M64.data.view(np.int64))),
-                      "reported_pattern_equal": bool(np.array_equal(Mref.indptr, M.indptr)
-                                                     and np.array_equal(Mref.indices, M.indices)),
-                      "fro_AM_minus_I_oracle": _fro_residual(prob.A, Mref, prob.B)}
-        what = (f"stratified sample of {ncols} of {n} columns (seed 3)" if sampled
-                else f"full workload (n={n})")
-        cpu = {"value": ncols / cpu_t, "unit": "columns/s", "cores": cores, "kind": "port",
-               "sample": f"{what} x {reps} runs on {cores} threads; 1 thread: {1.0 / cpu1:.0f} columns/s",
-               "single_thread_value": 1.0 / cpu1}
+        sample = 8192 if args.workload in ("random", "varcoef3d") else None
+        cpu, parity = _cpu_leg(prob, M64, M if rmx is not None else None, sample)
 
-    value = n / (ms_per_step * 1e-3)
+    value = n / (main_t["ms_per_step"] * 1e-3)
+    near = np.sum([x["near"] for x in c64["launches"]], axis=0)
     out = {
         "metric": "SPAI columns/s, mixed vs FP64 GPU at matched ||AM-I||_F; bwd/fwd time",
         "value": value,
@@ -352,32 +436,38 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_t["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 QR + f64 refinement" if rmx is not None else "f64",
         "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload][1].format(nx=nx, n=n),
+        "config": {"workload": WORKLOADS[args.workload][1].format(nx=size, n=n),
                    "n_columns": n, "nnz_A": int(prob.A.nnz), "nnz_M": int(M.nnz),
                    "precision": "mixed" if rmx is not None else "fp64",
                    "parallelism": "single GPU, persistent column queue"},
-        "device_ms_per_build": main_run["dev_ms"],
+        "device_ms_per_build": main_t["dev_ms"],
         "fp64_columns_per_s": n / (r64["ms_per_step"] * 1e-3),
+        "fp64_ms_per_build": r64["ms_per_step"],
         "fp64_device_ms_per_build": r64["dev_ms"],
         "mixed_columns_per_s": (n / (rmx["ms_per_step"] * 1e-3)) if rmx is not None else None,
         "mixed_over_fp64": (r64["ms_per_step"] / rmx["ms_per_step"]) if rmx is not None else None,
         "mixed_check": mixed_check,
         "fro_AM_minus_I": fro_gpu,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kern_name, "kernel_ms": kern_ms, "all_bins_kernel_ms": main_run["kern_ms"],
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "qr_flop_per_launch": float(work[0] + work[1])},
-        "bins": main_run["bins"],
+        "near_margin_decisions_fp64": {"score_at_mean": int(near[0]), "maxnew_cut": int(near[1]),
+                                       "residual_at_eps": int(near[2]),
+                                       "margin": "at most 1e-12 relative"},
+        "roofline": {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_kernel": traffic_kernel, **{k: v for k, v in dom.items() if k != "achieved"}},
+        "launches": _launch_table(main_t, main_c),
+        "launches_fp64": _launch_table(r64, c64),
+        "bins": main_t["bins"],
         "autotune": tune,
         "autotune_mixed": tune_mixed,
-        "fp64_with_cache": cache_run,
+        "binned_vs_unbinned": unbinned,
+        "cache": cache_run,
+        "probing3d": probing,
         "bwd_fwd": bwd,
         "block_spai": block,
         "cpu_baseline": cpu,

@@ -54,6 +54,8 @@ typedef struct {
     int binning;           /* GPU: 1 = default size bins, 2 = autotuned bins (mspai_autotune),
                               0 = one generic bin (1) */
     int verbose;           /* -vb */
+    int counters;          /* GPU: 1 = store the 8 work counters of every column (profiling,
+                              mspai_work_totals); 0 = off, no counter traffic (0) */
 } mspai_opts;
 
 /* Fill `o` with the reference's defaults. */
@@ -118,8 +120,9 @@ int mspai_result_nnz(const mspai_ctx* ctx, long long* nnz);
 int mspai_download_csc(mspai_ctx* ctx, int* colptr, int* rowind, double* val);
 
 /* Per-column results: final residual norm ||A(I,J) m - b(I)||, number of
- * least-squares solves, status bits, and 8 work counters per column.
- * Any pointer may be NULL. */
+ * least-squares solves, status bits, and 8 work counters per column (only
+ * after a build with opts->counters set; else MSPAI_ERR_STATE when `work`
+ * is not NULL).  Any pointer may be NULL. */
 int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int* nsolve, int* status,
                                double* work);
 
@@ -166,11 +169,37 @@ typedef struct {
 } mspai_tune_report;
 int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeats, mspai_tune_report* report);
 
-/* Sums over all columns of the 8 work counters of the last build:
+/* Sums over all columns of the 8 work counters of the last build (which
+ * must have run with opts->counters set; else MSPAI_ERR_STATE):
  * QR flop, other solve flop (Q^T b, triangular solve, residual), gather
  * bytes, candidate-union bytes, rho bytes, candidates scored, sum |I|,
  * sum |J| (over all least-squares solves). */
 int mspai_work_totals(mspai_ctx* ctx, double* totals);
+
+/*
+ * Per-launch profile of the last build or adjoint (no counterpart in the
+ * reference): for every column-kernel launch in order, its name (precision,
+ * size bin, lane-group width and |J|/|I| capacities, or the generic kernel),
+ * its device time from HIP events on the library's stream, and -- after a
+ * build with opts->counters set, else -1 -- the number of columns it
+ * finished, the sums of their 8 work counters (as mspai_work_totals) and of
+ * their near-margin decisions: FP64 selection decisions whose margin is at
+ * most 1e-12 relative, so that another operation order could decide them
+ * the other way (near[0]: a score against the mean threshold; near[1]: the
+ * cut after the maxnew-th score, ties across it included; near[2]: ||r||
+ * against epsilon), and the nonzeros of M they wrote.  A column that moves
+ * on to a later launch is counted there only.  Writes min(max, launches)
+ * entries and the count to *nlaunch.
+ */
+typedef struct {
+    char name[48];
+    double ms;
+    double columns;
+    double work[8];
+    double near[3];
+    double nnz;
+} mspai_launch_stat;
+int mspai_launch_stats(mspai_ctx* ctx, int max, int* nlaunch, mspai_launch_stat* out);
 
 /* Size-bin routing of the last build: nbins, and for b = 0..nbins the
  * columns routed to bin b up front and the columns it processed (routed plus

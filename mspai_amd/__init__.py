@@ -23,16 +23,22 @@ EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspa
             "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_reuse_pattern",
             "mspai_set_target",
             "mspai_build", "mspai_result_nnz", "mspai_download_csc",
-            "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts",
+            "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts", "mspai_launch_stats",
             "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
             "mspai_backward", "mspai_autotune", "mspai_apply",
             "mspai_set_block_size", "mspai_block_info", "mspai_block_sizes"]
 
 
+class LaunchStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("ms", C.c_double), ("columns", C.c_double),
+                ("work", C.c_double * 8), ("near", C.c_double * 3), ("nnz", C.c_double)]
+
+
 class Opts(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("maxnew", C.c_int), ("max_impr_steps", C.c_int),
                 ("use_mean", C.c_int), ("qr", C.c_int), ("cache", C.c_int),
-                ("precision", C.c_int), ("binning", C.c_int), ("verbose", C.c_int)]
+                ("precision", C.c_int), ("binning", C.c_int), ("verbose", C.c_int),
+                ("counters", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -168,10 +174,13 @@ class Builder:
                                             _p(_f64(B.data))), "mspai_set_target")
 
     def build(self, epsilon=0.4, maxnew=5, max_impr_steps=5, use_mean=1, qr=0, cache=0,
-              precision=0, binning=1, verbose=0):
-        o = Opts(epsilon, maxnew, max_impr_steps, use_mean, qr, cache, precision, binning, verbose)
+              precision=0, binning=1, verbose=0, counters=0):
+        """counters=1 stores the 8 work counters of every column (work_totals,
+        column_info()["work"]); off by default, as in timed builds."""
+        o = Opts(epsilon, maxnew, max_impr_steps, use_mean, qr, cache, precision, binning, verbose, counters)
         st = Stats()
         self._chk(self.lib.mspai_build(self.h, C.byref(o), C.byref(st)), "mspai_build")
+        self.counters = bool(counters)
         return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz,
                 "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow,
                 "n_cache_hits": st.n_cache_hits, "first_bin_ms": st.first_bin_ms}
@@ -223,6 +232,17 @@ class Builder:
         self._chk(self.lib.mspai_work_totals(self.h, _p(w)), "mspai_work_totals")
         return w
 
+    def launch_stats(self):
+        """Per column-kernel launch of the last build: name, device ms (HIP
+        events), and with counters on the columns it finished, their work
+        counter sums and near-margin decision counts (else -1)."""
+        buf = (LaunchStat * 16)()
+        n = C.c_int()
+        self._chk(self.lib.mspai_launch_stats(self.h, 16, C.byref(n), buf), "mspai_launch_stats")
+        return [{"name": buf[k].name.decode(), "ms": buf[k].ms, "columns": buf[k].columns,
+                 "work": list(buf[k].work), "near": list(buf[k].near), "nnz": buf[k].nnz}
+                for k in range(min(n.value, 16))]
+
     def bin_counts(self):
         nb = C.c_int()
         routed = np.zeros(8, np.int32)
@@ -246,10 +266,11 @@ class Builder:
         res = np.zeros(n, np.float64)
         ns = np.zeros(n, np.int32)
         stt = np.zeros(n, np.int32)
-        wk = np.zeros(n * NWORK, np.float64)
+        wk = np.zeros(n * NWORK, np.float64) if getattr(self, "counters", False) else None
         self._chk(self.lib.mspai_download_column_info(self.h, _p(res), _p(ns), _p(stt), _p(wk)),
                   "mspai_download_column_info")
-        return {"res_norm": res, "nsolve": ns, "status": stt, "work": wk.reshape(n, NWORK)}
+        return {"res_norm": res, "nsolve": ns, "status": stt,
+                "work": wk.reshape(n, NWORK) if wk is not None else None}
 
 
 def block_sizes(A, bs: int) -> np.ndarray:
@@ -269,7 +290,8 @@ def block_sizes(A, bs: int) -> np.ndarray:
 
 
 def spai(prob, device: int = 0, **opts):
-    """Build M for a mspai_amd.gen.Problem on the GPU.
+    """Build M for a mspai_amd.gen.Problem on the GPU (work counters on
+    unless counters=0 is passed).
     Returns (M as scipy CSC, per-column info, build stats)."""
     b = Builder(device)
     try:
@@ -277,12 +299,12 @@ def spai(prob, device: int = 0, **opts):
         b.set_pattern(prob.P)
         b.set_target(prob.B)
         kw = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
-                  use_mean=prob.use_mean)
+                  use_mean=prob.use_mean, counters=1)
         kw.update(opts)
         stats = b.build(**kw)
         M = b.download()
         info = b.column_info()
-        stats["work"] = b.work_totals()
+        stats["work"] = b.work_totals() if kw["counters"] else None
         stats["bins"] = b.bin_counts()
     finally:
         b.close()

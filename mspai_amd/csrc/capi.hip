@@ -25,6 +25,7 @@
 using namespace mspai;
 
 constexpr int kMaxBins = 4;
+constexpr int kMaxLaunch = 2 * kMaxBins + 2;  // column-kernel launches of one build: mixed bins, FP64 bins, generic
 
 namespace {
 
@@ -100,10 +101,12 @@ struct mspai_ctx {
     // start pattern
     int *d_p_ptr = nullptr, *d_p_ind = nullptr;
     int plen_max = 0;
-    bool pattern_set = false;
+    int p_cols = 0;             // columns the start pattern was set for
+    bool pattern_set = false;   // set by the caller (the implicit diagonal of a build is not)
     // target
     int *d_b_ptr = nullptr, *d_b_ind = nullptr;
     double* d_b_val = nullptr;
+    int b_rows = 0, b_cols = 0;  // shape the target was set for
     // outputs
     int cap = 0;
     size_t out_cols = 0;
@@ -136,6 +139,13 @@ struct mspai_ctx {
     bool a_csr_ok = false, m_csr_ok = false;
     bool ajn_ok = false;  // d_ajn holds the column norms of the current values
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // [4..5]: first size bin
+    /* per-launch profile of the last build: events around every column-kernel
+     * launch, and (builds with counters) the totals of the columns each finished */
+    hipEvent_t lev[2 * kMaxLaunch] = {};
+    int nlaunch = 0;
+    char lname[kMaxLaunch][48] = {};
+    double* d_lwork = nullptr;  // kMaxLaunch x kLaunchStat
+    bool lwork_valid = false;
     /* pattern->R cache */
     int cache_slots = 0, cache_capI = 0, cache_capJ = 0;
     int* d_cache_state = nullptr;
@@ -149,6 +159,7 @@ struct mspai_ctx {
     int last_nbins = 0;
     bool last_mixed = false;
     int last_binning = 1;
+    bool last_counters = false;  // the last build stored per-column work counters
     long long nnzB = 0;
     int last_routed[8] = {0};
     int mixed_fp64_last = -1;  // FP64 queue length of the last mixed build (-1: none since the inputs changed)
@@ -239,6 +250,7 @@ extern "C" void mspai_default_opts(mspai_opts* o)
     o->precision = 0;
     o->binning = 1;
     o->verbose = 0;
+    o->counters = 0;
 }
 
 extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
@@ -262,9 +274,15 @@ extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
             delete ctx;
             return MSPAI_ERR_HIP;
         }
+    for (auto& e : ctx->lev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return MSPAI_ERR_HIP;
+        }
     if (hipMalloc((void**)&ctx->d_counter, 64 * sizeof(int)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_counter, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_wsum, kNumWork * sizeof(double)) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_wsum, kNumWork * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_lwork, kMaxLaunch * kLaunchStat * sizeof(double)) != hipSuccess) {
         delete ctx;
         return MSPAI_ERR_HIP;
     }
@@ -302,6 +320,7 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_counter);
     if (ctx->h_counter) (void)hipHostFree(ctx->h_counter);
     dfree(ctx->d_wsum);
+    dfree(ctx->d_lwork);
     dfree(ctx->d_scratch);
     dfree(ctx->d_tie);
     dfree(ctx->d_cache_state);
@@ -320,6 +339,8 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_csr_cursor);
     block_free(ctx);
     for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->lev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -367,6 +388,7 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     int maxr = 0;
     for (int i = 0; i < nrows; ++i) maxr = std::max(maxr, rowptr[i + 1] - rowptr[i]);
     if (rowptr[nrows] != nnz) return fail(ctx, MSPAI_ERR_ARG, "upload: row pattern nnz differs from colptr");
+    const bool reshaped = nrows != ctx->nrows || ncols != ctx->ncols;
     int rc;
     if ((rc = upload(ctx, ctx->d_colptr, colptr, ncols + 1))) return rc;
     if ((rc = upload(ctx, ctx->d_rowind, rowind, nnz))) return rc;
@@ -390,10 +412,33 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
         block_free(ctx);
         ctx->pattern_set = false;
     }
-    if (!ctx->pattern_set) {
+    if (!ctx->pattern_set || reshaped) {
+        /* a start pattern or target of another shape would be read past its end */
         dfree(ctx->d_p_ptr);
         dfree(ctx->d_p_ind);
+        ctx->pattern_set = false;
+        ctx->p_cols = 0;
     }
+    if (reshaped) {
+        dfree(ctx->d_b_ptr);
+        dfree(ctx->d_b_ind);
+        dfree(ctx->d_b_val);
+        ctx->nnzB = 0;
+        ctx->b_rows = ctx->b_cols = 0;
+    }
+    return MSPAI_OK;
+}
+
+/* The start pattern and the target must have been set for the current
+ * matrix (columns of C, or block columns in block mode; B with C's shape). */
+static int check_shapes(mspai_ctx* ctx)
+{
+    const int pc = ctx->block_bs != 1 ? ctx->nb : ctx->ncols;
+    if (ctx->d_p_ptr && ctx->p_cols != pc)
+        return fail(ctx, MSPAI_ERR_STATE, "start pattern was set for " + std::to_string(ctx->p_cols) +
+                                              " columns, the matrix has " + std::to_string(pc));
+    if (ctx->d_b_ptr && (ctx->b_rows != ctx->nrows || ctx->b_cols != ctx->ncols))
+        return fail(ctx, MSPAI_ERR_STATE, "target was set for another matrix shape");
     return MSPAI_OK;
 }
 
@@ -441,6 +486,7 @@ extern "C" int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* r
     if ((rc = upload(ctx, ctx->d_p_ind, rowind, (size_t)colptr[n]))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->plen_max = pmax;
+    ctx->p_cols = n;
     ctx->pattern_set = true;
     return MSPAI_OK;
 }
@@ -475,11 +521,15 @@ extern "C" int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* ro
 {
     if (!ctx) return MSPAI_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    /* M and the FP64 queue length of the last mixed build belong to the old target */
+    ctx->nnzM = -1;
+    ctx->mixed_fp64_last = -1;
     if (!colptr) {
         dfree(ctx->d_b_ptr);
         dfree(ctx->d_b_ind);
         dfree(ctx->d_b_val);
         ctx->nnzB = 0;
+        ctx->b_rows = ctx->b_cols = 0;
         return MSPAI_OK;
     }
     if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "set_target: upload the matrix first");
@@ -497,6 +547,8 @@ extern "C" int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* ro
     if ((rc = upload(ctx, ctx->d_b_val, val, (size_t)colptr[n]))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nnzB = colptr[n];
+    ctx->b_rows = ctx->nrows;
+    ctx->b_cols = n;
     return MSPAI_OK;
 }
 
@@ -758,6 +810,31 @@ static int waves_per_cu(const Bin& b, int mode)
     return w;
 }
 
+/* Per-launch profile: an event before and after each column-kernel launch,
+ * the launch's name, and its slot of the counter totals (builds with
+ * counters only; a.work is set exactly then). */
+static int begin_launch(mspai_ctx* ctx, BuildArgs& a, const char* what, int mode, int bin, const Bin* b)
+{
+    const int k = ctx->nlaunch;
+    if (k >= kMaxLaunch) return fail(ctx, MSPAI_ERR_STATE, "more column-kernel launches than the profile holds");
+    static const char* kModes[] = {"FP64", "mixed", "adjoint FP64", "adjoint mixed", "FP64 QR update"};
+    if (b)
+        std::snprintf(ctx->lname[k], sizeof ctx->lname[k], "%s bin %d G=%d |J|<=%d |I|<=%d", kModes[mode], bin,
+                      b->group, b->L.capJ, b->L.capI);
+    else
+        std::snprintf(ctx->lname[k], sizeof ctx->lname[k], "%s %s", kModes[mode], what);
+    a.lwork = a.work ? ctx->d_lwork + (size_t)k * kLaunchStat : nullptr;
+    HIPCHK(hipEventRecord(ctx->lev[2 * k], ctx->stream));
+    return MSPAI_OK;
+}
+
+static int end_launch(mspai_ctx* ctx)
+{
+    HIPCHK(hipEventRecord(ctx->lev[2 * ctx->nlaunch + 1], ctx->stream));
+    ctx->nlaunch++;
+    return MSPAI_OK;
+}
+
 /* Most FP64 columns of a mixed build that the generic kernel takes alone. */
 constexpr int kDirectFp64 = 4096;
 
@@ -785,7 +862,11 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     int* mheads = ctx->d_counter + 32;
     int* fq = ctx->d_queues;
     int* mq = ctx->d_queues + (size_t)(kMaxBins + 1) * n;
+    int rc;
     HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 64 * sizeof(int), ctx->stream));
+    ctx->nlaunch = 0;
+    ctx->lwork_valid = a.work != nullptr;
+    if (a.work) HIPCHK(hipMemsetAsync(ctx->d_lwork, 0, kMaxLaunch * kLaunchStat * sizeof(double), ctx->stream));
     a.status_or = 0;
     /* HBM tie-path buffers of the LDS bins: one region per resident group */
     size_t tie_need = 0;
@@ -813,9 +894,11 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
             const ScratchLayout& L = bins[b].L;
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+            if ((rc = begin_launch(ctx, a, nullptr, fp64_mode == 2 ? 3 : 1, b, &bins[b]))) return rc;
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
                           ctx->num_cu * per_cu, ctx->stream);
             HIPCHK(hipGetLastError());
+            if ((rc = end_launch(ctx))) return rc;
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
         }
         a.status_or = COL_FALLBACK;
@@ -828,9 +911,10 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             a.n_fallback = nullptr;
             a.overflow = fq;  // never written: the generic layout holds every column
             a.n_overflow = counts + 7;
+            if ((rc = begin_launch(ctx, a, "generic (HBM working set)", fp64_mode, -1, nullptr))) return rc;
             launch_fp64_generic(a, G, fp64_mode, fq, counts, heads + nb, gwaves, ctx->stream);
             HIPCHK(hipGetLastError());
-            return MSPAI_OK;
+            return end_launch(ctx);
         }
     } else {
         launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->d_counter + 40,
@@ -847,16 +931,20 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         const ScratchLayout& L = bins[b].L;
         const bool first = b == 0 && !mixed;
         if (first) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+        if ((rc = begin_launch(ctx, a, nullptr, fp64_mode, b, &bins[b]))) return rc;
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());
+        if ((rc = end_launch(ctx))) return rc;
         if (first) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     }
     a.overflow = fq;  // never written: the generic layout holds every column
     a.n_overflow = counts + 7;
     if (nb == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));  // no bins: the generic kernel is the first
+    if ((rc = begin_launch(ctx, a, "generic (HBM working set)", fp64_mode, -1, nullptr))) return rc;
     launch_fp64_generic(a, G, fp64_mode, fq + (size_t)nb * n, counts + nb, heads + nb, gwaves, ctx->stream);
     HIPCHK(hipGetLastError());
+    if ((rc = end_launch(ctx))) return rc;
     if (nb == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     return MSPAI_OK;
 }
@@ -954,9 +1042,11 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     int rc;
     if ((rc = check_opts(ctx, opts))) return rc;
     HIPCHK(hipSetDevice(ctx->device));
+    if ((rc = check_shapes(ctx))) return rc;
     if (ctx->block_bs != 1) return build_block(ctx, opts, stats);
     if (!ctx->d_p_ptr) {
         if ((rc = mspai_set_pattern(ctx, nullptr, nullptr))) return rc;
+        ctx->pattern_set = false;  // the implicit diagonal: rebuilt for the next matrix
     }
     const int n = ctx->ncols;
     const int ns = opts->max_impr_steps;
@@ -996,7 +1086,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     a.res = ctx->d_res;
     a.nsolve = ctx->d_nsolve;
     a.status = ctx->d_status;
-    a.work = ctx->d_work;
+    a.work = opts->counters ? ctx->d_work : nullptr;  // opt-in: 64 B per column of writes
     const bool mixed = opts->precision == 1 && !bins.empty() && !qrup;
     if (opts->cache > 0 && !qrup && opts->precision == 0) {
         const int cI = std::min(G.capI, 32), cJ = std::min(cap, 16);
@@ -1036,6 +1126,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     ctx->last_mixed = mixed;
     ctx->mixed_fp64_last = mixed ? ctx->h_counter[0] : -1;
     ctx->last_binning = opts->binning;
+    ctx->last_counters = opts->counters != 0;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float t_all = 0, t_solve = 0;
@@ -1098,6 +1189,7 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
     if (d_gB && !ctx->d_b_ptr) return fail(ctx, MSPAI_ERR_ARG, "backward: gB given but no target set");
     HIPCHK(hipSetDevice(ctx->device));
     int rc;
+    if ((rc = check_shapes(ctx))) return rc;
     const int n = ctx->ncols;
     const int cap = ctx->cap;
     const int needs = kNeedBackward | (opts->precision == 1 ? kNeedMixed : 0);
@@ -1185,12 +1277,39 @@ extern "C" int mspai_work_totals(mspai_ctx* ctx, double* totals)
 {
     if (!ctx || !totals) return MSPAI_ERR_ARG;
     if (ctx->nnzM < 0) return fail(ctx, MSPAI_ERR_STATE, "work_totals: build first");
+    if (!ctx->last_counters)
+        return fail(ctx, MSPAI_ERR_STATE, "work_totals: the last build ran with counters off");
     HIPCHK(hipSetDevice(ctx->device));
     hipLaunchKernelGGL(work_sum_kernel, dim3(kNumWork), dim3(256), 0, ctx->stream, ctx->d_work, ctx->ncols,
                        ctx->d_wsum);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(totals, ctx->d_wsum, kNumWork * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_launch_stats(mspai_ctx* ctx, int max, int* nlaunch, mspai_launch_stat* out)
+{
+    if (!ctx || !nlaunch || (max > 0 && !out)) return MSPAI_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *nlaunch = ctx->nlaunch;
+    std::vector<double> tot((size_t)kMaxLaunch * kLaunchStat, -1.0);
+    if (ctx->lwork_valid)
+        HIPCHK(hipMemcpy(tot.data(), ctx->d_lwork, tot.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < ctx->nlaunch && k < max; ++k) {
+        mspai_launch_stat& o = out[k];
+        std::memset(&o, 0, sizeof o);
+        std::snprintf(o.name, sizeof o.name, "%s", ctx->lname[k]);
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->lev[2 * k], ctx->lev[2 * k + 1]));
+        o.ms = ms;
+        const double* t = tot.data() + (size_t)k * kLaunchStat;
+        for (int q = 0; q < kNumWork; ++q) o.work[q] = t[q];
+        o.columns = t[kNumWork];
+        for (int q = 0; q < 3; ++q) o.near[q] = t[kNumWork + 1 + q];
+        o.nnz = t[kNumWork + 4];
+    }
     return MSPAI_OK;
 }
 
@@ -1234,6 +1353,8 @@ extern "C" int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int*
     if (res_norm) HIPCHK(hipMemcpy(res_norm, ctx->d_res, n * sizeof(double), hipMemcpyDeviceToHost));
     if (nsolve) HIPCHK(hipMemcpy(nsolve, ctx->d_nsolve, n * sizeof(int), hipMemcpyDeviceToHost));
     if (status) HIPCHK(hipMemcpy(status, ctx->d_status, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (work && !ctx->last_counters)
+        return fail(ctx, MSPAI_ERR_STATE, "download: the last build ran with counters off");
     if (work) HIPCHK(hipMemcpy(work, ctx->d_work, n * kNumWork * sizeof(double), hipMemcpyDeviceToHost));
     return MSPAI_OK;
 }
@@ -1655,7 +1776,9 @@ static int build_block(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stat
 
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->d_counter + 56, 0, 2 * sizeof(int), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_work, 0, (size_t)n * kNumWork * sizeof(double), ctx->stream));
+    ctx->last_counters = false;  // the block kernel keeps no work counters
+    ctx->nlaunch = 0;            // nor a per-launch profile
+    ctx->lwork_valid = false;
     if (ns > 0) launch_block_sq_inverse(a, ctx->d_sqtmp, ctx->stream);
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     launch_block_columns(a, L, (int)nwaves, ctx->stream);

@@ -26,6 +26,12 @@ enum ColStatus : int {
 
 constexpr int kWave = 64;
 constexpr int kNumWork = 8;
+constexpr int kLaunchStat = 13;  // per-launch totals: kNumWork counters, columns, 3 near-margin counts, nnz
+/* A selection decision of the FP64 path whose margin is at most kNearMargin
+ * relative (a score against the mean, the maxnew cut, ||r|| against eps) is
+ * counted as near: another operation order (MKL's) could decide it the
+ * other way. */
+constexpr double kNearMargin = 1e-12;
 
 /* Unrolled |J| width of the register-resident QR for a lane-group width. */
 __host__ __device__ constexpr int register_width(int group) { return group == 16 ? 8 : (group == 32 ? 16 : 32); }
@@ -110,6 +116,9 @@ struct BuildArgs {
     int* nsolve;
     int* status;
     double* work;
+    double* lwork;          // builds with counters: totals of this launch's finished columns
+                            // (kLaunchStat doubles: the 8 work counters, columns, near-margin
+                            // decisions at the mean, at the maxnew cut, at epsilon, nnz of M)
     char* scratch;          // per-wave HBM working sets of the generic kernel
     char* tie_scratch;      // per-group HBM tie-path buffers of the LDS bins
 };

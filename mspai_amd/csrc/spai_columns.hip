@@ -1275,7 +1275,7 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
  * are rare: none in the five workloads, a handful in the CSparse tests). */
 template <int G, bool kMixed, bool kTies>
 __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int col,
-                                       int m, int n, double rn, int* st, double* wk)
+                                       int m, int n, double rn, int* st, double* wk, int* near = nullptr)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
@@ -1342,6 +1342,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         return -1;
     }
     /* rho_j = sqrt(||r||^2 - (r^T a_j)^2 / ||a_j||^2), numerator over I only */
+    bool rho_near0 = false;  // mixed path: rho^2 within rounding of 0 (its sign decides NaN or not)
     for (int t = lane; t < c; t += G) {
         int j = s.jt[t];
         double num = 0.0;
@@ -1366,10 +1367,12 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         num = __dmul_rn(num, num);
         double ajn = a.ajn[j];
         double rho = __dsub_rn(__dmul_rn(rn, rn), __ddiv_rn(num, __dmul_rn(ajn, ajn)));
+        if (kMixed) rho_near0 |= rho <= kGuard * rn * rn;
         s.rho[t] = sqrt(rho);
         acc4 += 16.0 + 12.0 * (e1 - e0);
     }
     __syncthreads();
+    if (kMixed && Gr::ballot(rho_near0) != 0ull) return kFallback;
     acc4 = Gr::sum_fixed(acc4);
     double mean = 0.0;
     if (lane == 0) {
@@ -1386,6 +1389,12 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         if (kMixed) return kFallback;
         *st |= COL_NAN_RHO;
         return n;
+    }
+    if (!kMixed && near) {
+        /* scores within kNearMargin of the mean threshold */
+        int cnt = 0;
+        for (int t = lane; t < c; t += G) cnt += fabs(s.rho[t] - mean) <= kNearMargin * fabs(mean);
+        near[0] += Gr::sum_int(cnt);
     }
     if (kMixed) {
         /* no score within the guard band of the mean threshold */
@@ -1421,6 +1430,15 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         prev = v;
     }
     int take_ties = sneed - below;
+    if (!kMixed && near && q > sneed) {
+        /* the cut after the sneed-th score: a tie across it, or the next
+         * score within kNearMargin */
+        double vnext = INFINITY;
+        for (int t = lane; t < c; t += G)
+            if (s.rho[t] > vstar) vnext = fmin(vnext, s.rho[t]);
+        vnext = Gr::min(vnext);
+        near[1] += (take_ties != ties || vnext - vstar <= kNearMargin * vstar) ? 1 : 0;
+    }
     if (kMixed && q > sneed) {
         /* the cut after the sneed-th score must be clear of the next score */
         if (take_ties != ties) return kFallback;
@@ -1473,6 +1491,21 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
     }
     __syncthreads();
     return n + sneed;
+}
+
+/* Builds with counters: add a finished column's work counters, near-margin
+ * counts and |J| to its launch's totals (one atomic per counter). */
+__device__ __forceinline__ void launch_totals(const BuildArgs& a, const double* wsum, const int* near, int lane,
+                                              int nnz)
+{
+    if (lane < kNumWork) atomicAdd(&a.lwork[lane], wsum[lane]);
+    if (lane == 0) {
+        atomicAdd(&a.lwork[kNumWork], 1.0);
+        atomicAdd(&a.lwork[kNumWork + 4], (double)nnz);
+        if (near)
+            for (int q = 0; q < 3; ++q)
+                if (near[q]) atomicAdd(&a.lwork[kNumWork + 1 + q], (double)near[q]);
+    }
 }
 
 template <int G>
@@ -1819,6 +1852,7 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
     }
     __syncthreads();
     if (a.work && lane < kNumWork) a.work[(size_t)col * kNumWork + lane] = s.wsum[lane];
+    if (a.lwork) launch_totals(a, s.wsum, nullptr, lane, n);
     return kDone;
 }
 
@@ -2000,6 +2034,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
     for (int i = lane; i < n; i += G) s.J[i] = a.p_ind[p0 + i];
     __syncthreads();
     int st = 0, solves = 0;
+    int near[3] = {0, 0, 0};  // near-margin decisions (counters builds, FP64)
     double rn = 1.0e50;
     for (int step = -1; step < a.ns; ++step) {
         int m = build_I<G>(a, L, s, n);
@@ -2024,8 +2059,9 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         count_solve<G>(a, s, m, n, wk);
         rn = residual_norm<G>(a, L, s, m, n, dense);
         if (kMixed && !(fabs(rn - a.epsilon) > kGuard * a.epsilon)) return kRedoFp64;
+        if (!kMixed && wk) near[2] += fabs(rn - a.epsilon) <= kNearMargin * a.epsilon;
         if (rn <= a.epsilon || step == a.ns - 1) break;
-        int nn = augment<G, kMixed, kTies>(a, L, s, col, m, n, rn, &st, wk);
+        int nn = augment<G, kMixed, kTies>(a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
         if (nn == kOverflow) return kMoveOn;
         if (nn == kFallback) return kRedoFp64;
         if (nn < 0) break;
@@ -2044,6 +2080,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
     }
     __syncthreads();
     if (a.work && lane < kNumWork) a.work[(size_t)col * kNumWork + lane] = s.wsum[lane];
+    if (a.lwork) launch_totals(a, s.wsum, near, lane, n);
     return kDone;
 }
 

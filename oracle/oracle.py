@@ -70,7 +70,9 @@ def capacity(prob) -> int:
 def spai(prob, cols=None, threads: int = 1, qr: int = 0, cache: int = 0, epsilon=None,
          maxnew=None, max_impr_steps=None, use_mean=None):
     """Run the oracle on `prob` (a mspai_amd.gen.Problem); returns a dict with
-    len, idx, val (padded to `cap`), res_norm, nsolve, status, work, cap."""
+    len, idx, val (padded to `cap`), res_norm, nsolve, status, work, cap, and
+    near (per column: near-margin decisions at the mean, at the maxnew cut,
+    at epsilon; fresh-QR path only)."""
     L = lib()
     keep = []
     A = _csc_struct(prob.A, keep)
@@ -98,15 +100,16 @@ def spai(prob, cols=None, threads: int = 1, qr: int = 0, cache: int = 0, epsilon
     nsolve = np.zeros(ncl, np.int32)
     status = np.zeros(ncl, np.int32)
     work = np.zeros(ncl * NWORK, np.float64)
+    near = np.zeros(ncl * 3, np.int32)
     rc = L.orc_spai(C.byref(A), _p(rp_ptr), _p(rp_ind), _p(p_ptr), _p(p_ind),
                     C.byref(B) if B is not None else None, C.byref(o), _p(colsarr), C.c_int(ncl),
                     C.c_int(cap), _p(m_len), _p(m_idx), _p(m_val), _p(res), _p(nsolve), _p(status),
-                    _p(work))
+                    _p(work), _p(near))
     if rc != 0:
         raise RuntimeError(f"orc_spai failed: {rc}")
     return {"len": m_len, "idx": m_idx.reshape(ncl, cap), "val": m_val.reshape(ncl, cap),
             "res_norm": res, "nsolve": nsolve, "status": status,
-            "work": work.reshape(ncl, NWORK), "cap": cap}
+            "work": work.reshape(ncl, NWORK), "near": near.reshape(ncl, 3), "cap": cap}
 
 
 def to_csc(out, nrows):

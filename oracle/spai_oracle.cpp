@@ -189,7 +189,12 @@ struct Work {
     std::vector<double> ahat, aqr, tau, b, b0, r;
     std::vector<RhoIdx> rhos;
     std::list<CacheEntry> cache;
+    bool count_near = false;  // fresh-QR path: count near-margin decisions
+    int near[3] = {0, 0, 0};
 };
+
+/* Margin at or below which a selection decision counts as near. */
+constexpr double kNearMargin = 1e-12;
 
 struct Problem {
     const orc_csc* A;
@@ -347,10 +352,16 @@ bool augment(const Problem& P, Work& w, int col, double rn, bool* nan_mean, doub
     }
     if (work) work[5] += (double)w.Jt.size();
     if (P.o->use_mean) mean /= (double)w.Jt.size();
+    if (w.count_near && !std::isnan(mean))
+        for (const RhoIdx& c : w.rhos) w.near[0] += std::fabs(c.rho - mean) <= kNearMargin * std::fabs(mean);
     std::sort(w.rhos.begin(), w.rhos.end(), RhoCmp());
     *nan_mean = std::isnan(mean);
-    for (int el = 0; el < P.o->maxnew && el < (int)w.Jt.size() && w.rhos[el].rho <= mean; ++el)
+    int el = 0;
+    for (; el < P.o->maxnew && el < (int)w.Jt.size() && w.rhos[el].rho <= mean; ++el)
         w.J.push_back(w.rhos[el].idx);
+    /* the cut: another candidate within the threshold was left out */
+    if (w.count_near && el > 0 && el == P.o->maxnew && el < (int)w.Jt.size() && w.rhos[el].rho <= mean)
+        w.near[1] += w.rhos[el].rho - w.rhos[el - 1].rho <= kNearMargin * w.rhos[el - 1].rho;
     std::sort(w.J.begin(), w.J.end());
     return true;
 }
@@ -434,6 +445,7 @@ void spai_column(const Problem& P, Work& w, int col, int* len, int* idx, double*
         int m = (int)w.I.size(), n = (int)w.J.size();
         w.r.resize(m);
         rn = residual(m, n, w.ahat.data(), w.b.data(), w.b0.data(), w.r.data());
+        if (w.count_near) w.near[2] += std::fabs(rn - o->epsilon) <= kNearMargin * o->epsilon;
         if (rn <= o->epsilon || step == o->max_impr_steps - 1) break;
         bool nan_mean = false;
         if (!augment(P, w, col, rn, &nan_mean, work)) break;
@@ -633,7 +645,7 @@ void spai_column_qrupdate(const Problem& P, Work& w, int col, int* len, int* idx
 extern "C" int orc_spai(const orc_csc* A, const int* rp_ptr, const int* rp_ind, const int* p_ptr,
                         const int* p_ind, const orc_csc* B, const orc_opts* o, const int* cols,
                         int ncols_list, int cap, int* m_len, int* m_idx, double* m_val,
-                        double* res_norm, int* nsolve, int* status, double* work)
+                        double* res_norm, int* nsolve, int* status, double* work, int* near)
 {
     if (!A || !p_ptr || !p_ind || !o || cap <= 0) return -1;
     if (o->max_impr_steps > 0 && (!rp_ptr || !rp_ind)) return -2;
@@ -671,12 +683,18 @@ extern "C" int orc_spai(const orc_csc* A, const int* rp_ptr, const int* rp_ind, 
                 err = -4;
                 continue;
             }
-            if (o->qr == 2 && o->max_impr_steps > 0 && o->maxnew > 0)
+            w.near[0] = w.near[1] = w.near[2] = 0;
+            w.count_near = false;
+            if (o->qr == 2 && o->max_impr_steps > 0 && o->maxnew > 0) {
                 spai_column_qrupdate(P, w, col, &m_len[p], m_idx + (size_t)p * cap, m_val + (size_t)p * cap,
                                      &res_norm[p], &nsolve[p], &status[p], wk);
-            else
+            } else {
+                w.count_near = true;
                 spai_column(P, w, col, &m_len[p], m_idx + (size_t)p * cap, m_val + (size_t)p * cap,
                             &res_norm[p], &nsolve[p], &status[p], wk);
+            }
+            if (near)
+                for (int q = 0; q < 3; ++q) near[(size_t)p * 3 + q] = w.near[q];
         }
     };
     if (nt == 1) {

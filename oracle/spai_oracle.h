@@ -70,14 +70,18 @@ typedef struct {
  *   cap           : capacity per column of the output slots
  * Outputs (indexed by position in `cols`):
  *   m_len, m_idx[pos*cap ..], m_val[pos*cap ..], res_norm, nsolve, status,
- *   work[pos*ORC_NWORK ..] (may be NULL).
+ *   work[pos*ORC_NWORK ..] (may be NULL),
+ *   near[pos*3 ..] (may be NULL): near-margin decisions of the fresh-QR
+ *   path, margin at most 1e-12 relative: scores against the mean, the cut
+ *   after the maxnew-th score (ties across it included), ||r|| against eps
+ *   (zero on the QR-update path).
  * Returns 0, or a negative error code.
  */
 int orc_spai(const orc_csc* A, const int* rp_ptr, const int* rp_ind,
              const int* p_ptr, const int* p_ind, const orc_csc* B,
              const orc_opts* o, const int* cols, int ncols_list, int cap,
              int* m_len, int* m_idx, double* m_val, double* res_norm,
-             int* nsolve, int* status, double* work);
+             int* nsolve, int* status, double* work, int* near);
 
 /* Dense least squares min ||A x - b|| in the oracle's order.
  * A: column-major m x n (overwritten by R and the Householder vectors),

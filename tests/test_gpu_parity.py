@@ -13,7 +13,7 @@ import pytest
 import scipy.sparse as sp
 
 from mspai_amd import gen
-from tests.util import column_lstsq, csparse_problem, ls_close, same_csc_bitwise
+from tests.util import column_lstsq, csparse_problem, ls_close, same_csc_bitwise, same_pattern
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -110,6 +110,57 @@ def test_identity(oracle, gpu):
     ref, M, info, _ = run_both(oracle, gpu, prob)
     assert_bitwise(oracle, prob, ref, M, info)
     assert same_csc_bitwise(M, sp.identity(n, format="csc"))
+
+
+@pytest.mark.parametrize("a,nsolve", [(0.3736, 1), (0.5774, 2)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stop_rule_hand_cases(oracle, gpu, a, nsolve, precision):
+    """A = [[1, 0], [a, 1]], eps = 0.4: ||r|| after the first solve of column 0
+    is 0.35 (in (eps^2, eps]: stop) or 0.50 (in (eps, sqrt(eps)]: go on), so a
+    stop test on ||r||^2 or against eps^2 gives another solve count."""
+    A = sp.csc_matrix(np.array([[1.0, 0.0], [a, 1.0]]))
+    prob = gen.Problem("stop_rule", A, sp.csc_matrix(np.eye(2)), None, epsilon=0.4, maxnew=1,
+                       max_impr_steps=3)
+    ref, M, info, _ = run_both(oracle, gpu, prob, precision=precision)
+    assert int(info["nsolve"][0]) == nsolve
+    if precision == 0:
+        assert_bitwise(oracle, prob, ref, M, info)
+    else:
+        assert np.array_equal(ref["nsolve"], info["nsolve"])
+        assert np.array_equal(oracle.to_csc(ref, prob.n).indices, M.indices)
+
+
+def test_reupload_other_shape_on_one_builder(oracle, gpu):
+    """A start pattern or target set for one matrix is dropped when a matrix
+    of another shape is uploaded on the same context, so the build starts
+    from the diagonal again instead of reading the old arrays past their end;
+    an upload of the same shape keeps the caller's pattern."""
+    small, big = gen.config3(n=2048, bw=200), gen.config3(n=4096, bw=300)
+    b = gpu.Builder(0)
+    try:
+        b.upload(small.A, small.row_pattern())
+        b.set_pattern(gen.pattern_of(small.A))
+        b.build(epsilon=small.epsilon, maxnew=small.maxnew, max_impr_steps=small.max_impr_steps)
+        b.upload(big.A, big.row_pattern())
+        b.build(epsilon=big.epsilon, maxnew=big.maxnew, max_impr_steps=big.max_impr_steps)
+        ref = oracle.spai(big, threads=THREADS)
+        assert same_csc_bitwise(oracle.to_csc(ref, big.n), b.download())
+        # same shape: the caller's pattern stays
+        b.set_pattern(gen.pattern_of(big.A))
+        b.upload(big.A, big.row_pattern())
+        b.build(epsilon=big.epsilon, maxnew=0, max_impr_steps=0)
+        assert same_pattern(b.download(), gen.pattern_of(big.A))
+        # a target of another shape is dropped with the matrix
+        p4 = gen.config4(6)
+        b.upload(p4.A, p4.row_pattern())
+        b.set_pattern(p4.P)
+        b.set_target(p4.B)
+        b.upload(small.A, small.row_pattern())
+        b.build(epsilon=small.epsilon, maxnew=small.maxnew, max_impr_steps=small.max_impr_steps)
+        ref = oracle.spai(small, threads=THREADS)
+        assert same_csc_bitwise(oracle.to_csc(ref, small.n), b.download())
+    finally:
+        b.close()
 
 
 def test_binning_off_matches(oracle, gpu):
@@ -220,7 +271,7 @@ def test_autotuned_bins_bitwise(oracle, gpu):
     opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
     rep = b.autotune(repeats=1, **opts)
     print(f"[autotune] chose {rep['name']} from {rep['candidate_ms']}", flush=True)
-    b.build(binning=2, **opts)
+    b.build(binning=2, counters=1, **opts)
     M = b.download()
     info = b.column_info()
     b.close()
@@ -258,7 +309,7 @@ def test_new_values_adaptive_rebuild(oracle, gpu):
     assert np.array_equal(A2.indptr, prob.A.indptr) and np.array_equal(A2.indices, prob.A.indices)
     b.update_values(A2.data)
     b.set_pattern(prob.P)
-    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps, counters=1)
     M2 = b.download()
     info = b.column_info()
     b.close()
@@ -284,7 +335,7 @@ def test_reuse_pattern_static_resolve(oracle, gpu):
     A2.data = A2.data * (1.0 + 0.05 * rng.standard_normal(A2.nnz))
     b.update_values(A2.data)
     b.reuse_pattern()
-    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=0)
+    b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=0, counters=1)
     M2 = b.download()
     info = b.column_info()
     b.close()
@@ -293,3 +344,33 @@ def test_reuse_pattern_static_resolve(oracle, gpu):
     ref = oracle.spai(prob2, threads=THREADS)
     assert np.array_equal(M2.indptr, M1.indptr) and np.array_equal(M2.indices, M1.indices)
     assert_bitwise(oracle, prob2, ref, M2, info)
+
+
+@pytest.mark.parametrize("name", ["cs_fs_183_1", "cs_bcsstk01", "cs_ibm32a", "c3_random_8192"])
+def test_launch_stats_and_near_margins(oracle, gpu, name):
+    """Per-launch totals of a build with counters: the launches together
+    finish every column once, their work counters add up to the build's, and
+    their near-margin decision counts (scores at the mean, the maxnew cut,
+    ||r|| at eps) equal the oracle's, which makes the same decisions."""
+    prob = CASES[name]()
+    ref = oracle.spai(prob, threads=THREADS)
+    b = gpu.Builder(0)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps, counters=1)
+        ls = b.launch_stats()
+        tot = b.work_totals()
+        b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+        ls_off = b.launch_stats()
+    finally:
+        b.close()
+    print(f"[launches {name}] " + "; ".join(f"{x['name']}: {x['ms']:.3f} ms, {x['columns']:.0f} cols" for x in ls),
+          flush=True)
+    assert sum(x["columns"] for x in ls) == prob.n
+    assert sum(x["nnz"] for x in ls) == ref["len"].sum()
+    assert np.array_equal(np.sum([x["work"] for x in ls], axis=0), tot)
+    assert np.array_equal(np.sum([x["near"] for x in ls], axis=0), ref["near"].sum(axis=0))
+    assert all(x["ms"] >= 0 for x in ls)
+    assert [x["name"] for x in ls_off] == [x["name"] for x in ls]
+    assert all(x["columns"] == -1 for x in ls_off)

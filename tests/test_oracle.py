@@ -185,8 +185,75 @@ def test_adaptive_result_is_least_squares_optimal(oracle, make):
         assert ls_close(Ah, b, got, x), k
         rn = np.linalg.norm(Ah @ got - b)
         assert abs(out["res_norm"][k] - rn) <= 1e-12 * max(1.0, rn)
-        # stopping rule: either below eps, or out of steps, or no candidates left
-        assert rn <= p.epsilon or out["nsolve"][k] == p.max_impr_steps + 1 or out["len"][k] >= 1
+
+
+def _candidates_bruteforce(A, J, k):
+    """J~ = (union of the row patterns of I u {k}) minus J, from dense boolean
+    products (Augment_Sparsity, src/Spai_Sub.imp:1044-1069)."""
+    D = (sp.csc_matrix(A).toarray() != 0)
+    I = np.flatnonzero(D[:, J].any(axis=1))
+    L = np.union1d(I, [k])
+    U = np.flatnonzero(D[L, :].any(axis=0))
+    return np.setdiff1d(U, J)
+
+
+@pytest.mark.parametrize("make", [lambda: gen.config2(16), lambda: gen.config3(n=1024, bw=100),
+                                  lambda: csparse_problem("fs_183_1")])
+def test_stop_rule_is_norm_below_eps(oracle, make):
+    """The augmentation loop stops at the first step with ||r||_2 <= eps (the
+    norm, not its square), after ns augmentations, or when J~ is empty
+    (src/Spai_Unrestrained.imp:78-129).  J after s steps is read from a run
+    with ns = s; every step's solution is checked against numpy lstsq on that
+    step's (I, J) and its ||r|| recomputed with numpy.  Columns with ||r|| in (eps, sqrt(eps)] at a step where the
+    loop went on, and in (eps^2, eps] at the step where it stopped, are
+    required: a rule on ||r||^2 <= eps would stop at the first kind, a rule on
+    ||r|| <= eps^2 would go on at the second."""
+    p = make()
+    ns, eps = p.max_impr_steps, p.epsilon
+    runs = [oracle.spai(p, max_impr_steps=s) for s in range(ns + 1)]
+    full = runs[-1]
+    went_on_below_sqrt = stopped_above_sq = 0
+    for k in range(p.n):
+        t = int(full["nsolve"][k]) - 1  # augmentation steps taken
+        rns = []
+        for s in range(t + 1):
+            J = runs[s]["idx"][k, :runs[s]["len"][k]]
+            I, Ah, b, x = column_lstsq(p.A, J, k)
+            assert ls_close(Ah, b, runs[s]["val"][k, :J.size], x), (k, s)
+            rns.append(float(np.linalg.norm(Ah @ runs[s]["val"][k, :J.size] - b)))
+        for s in range(t):  # the loop went on: ||r|| above eps
+            if abs(rns[s] - eps) > 1e-12:
+                assert rns[s] > eps, (k, s, rns[s])
+                went_on_below_sqrt += rns[s] <= math.sqrt(eps)
+        rt = rns[t]
+        assert abs(full["res_norm"][k] - rt) <= 1e-12 * max(1.0, rt)
+        if abs(rt - eps) <= 1e-12:
+            continue
+        if rt <= eps:
+            stopped_above_sq += rt > eps * eps
+        else:  # out of steps, or no candidate left
+            J = full["idx"][k, :full["len"][k]]
+            assert t == ns or _candidates_bruteforce(p.A, J, k).size == 0, (k, t, rt)
+    assert went_on_below_sqrt >= 1 and stopped_above_sq >= 1, (went_on_below_sqrt, stopped_above_sq)
+
+
+@pytest.mark.parametrize("a,nsolve", [(0.3736, 1), (0.5774, 2)])
+def test_stop_rule_hand_cases(oracle, a, nsolve):
+    """A = [[1, 0], [a, 1]], J0 = {0}, eps = 0.4: column 0 has
+    ||r|| = |a| / sqrt(1 + a^2) after the first solve, 0.35 in (eps^2, eps]
+    for a = 0.3736 (stop: one solve) and 0.50 in (eps, sqrt(eps)] for
+    a = 0.5774 (go on: J~ = {1}, a second solve)."""
+    A = sp.csc_matrix(np.array([[1.0, 0.0], [a, 1.0]]))
+    P = sp.csc_matrix(np.eye(2))
+    p = gen.Problem("stop_rule", A, P, None, epsilon=0.4, maxnew=1, max_impr_steps=3)
+    out = oracle.spai(p)
+    rn0 = abs(a) / math.sqrt(1.0 + a * a)
+    assert (rn0 <= 0.4) == (nsolve == 1)
+    assert int(out["nsolve"][0]) == nsolve
+    if nsolve == 1:
+        assert abs(out["res_norm"][0] - rn0) <= 1e-15
+    else:
+        assert list(out["idx"][0, :out["len"][0]]) == [0, 1]
 
 
 def test_threads_are_deterministic(oracle):

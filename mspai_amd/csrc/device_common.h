@@ -96,6 +96,9 @@ struct BuildArgs {
     int* fallback;          // mixed path: columns to rebuild in FP64
     int* n_fallback;
     int status_or;          // status bits added to every column of this launch
+    int save_state;         // a column that outgrows this launch's layout leaves its state (J, step,
+                            // status, counters) for the next launch of the same precision to resume;
+                            // 0: the next launch starts it afresh (another precision follows)
     const double* m_in;     // adjoint: values of M (pattern in p_ptr/p_ind)
     const double* gM;       // adjoint: dL/dM, indexed like m_in
     double* gC;             // adjoint: dL/dC, indexed like A.val (accumulated)

@@ -66,6 +66,8 @@ struct WaveScratch {
     int* sid;      // flat entry -> position t in J
     double* wsum;  // the column's work counters
     double* rq;    // adjoint: R of the register QR (capJ x capJ)
+    unsigned* htab;  // row -> position hash of the candidate scores: always in LDS
+    int hslots;      // its slots (a power of two)
 };
 
 __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const ScratchLayout& L)
@@ -103,10 +105,13 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const S
     s.sid = (int*)(ws + L.sid);
     s.wsum = (double*)(ws + L.wsum);
     s.rq = (double*)(ws + L.rq);
+    s.htab = nullptr;  // set by the kernel (LDS)
+    s.hslots = 0;
     return s;
 }
 
 constexpr int kOverflow = -2;
+constexpr int kGenericHashSlots = 2048;  // LDS of the generic kernel's score hash (8 KB per wave)
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, 32 CUs each
 constexpr int kFallback = -3;
 
@@ -1267,6 +1272,60 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
     return sqrt(sum);
 }
 
+/* Row -> position in I for the candidate scores: open addressing (linear
+ * probing, Fibonacci hashing) over s.hslots slots of LDS -- in a size bin the
+ * right-hand-side copy b0, which is dead from the residual to the next
+ * right-hand side; in the generic kernel (working set in HBM) a region of
+ * LDS of its own, so that the table is never written by atomics in L2 and
+ * read back through the vector L1.  Entries pack the row above the
+ * position; at most half of the slots are used.  When I does not fit (or
+ * rows need more bits than the packing leaves) the lookups binary-search
+ * the sorted I; a probe sequence is also cut off after every slot. */
+struct RowHash {
+    unsigned* tab;  // nullptr: binary search
+    int bits, pbits;
+};
+
+template <int G>
+__device__ __forceinline__ RowHash row_hash_build(const WaveScratch& s, const ScratchLayout& L, int m, int nrows)
+{
+    const int lane = Group<G>::lane();
+    RowHash h{nullptr, 31 - __clz(s.hslots > 1 ? s.hslots : 1), 32 - __clz(L.capI > 2 ? L.capI - 1 : 1)};
+    const int nslots = 1 << h.bits;
+    if (!s.htab || 2 * m > nslots || (unsigned)nrows > (0xffffffffu >> h.pbits)) return h;
+    h.tab = s.htab;
+    for (int i = lane; i < nslots; i += G) h.tab[i] = 0xffffffffu;
+    __syncthreads();
+    for (int i = lane; i < m; i += G) {
+        const unsigned row = (unsigned)s.I[i];
+        unsigned slot = (row * 0x9e3779b1u) >> (32 - h.bits);
+        /* at least half of the slots stay empty: a free one is found */
+        for (int probe = 0; probe < nslots; ++probe) {
+            if (atomicCAS(&h.tab[slot], 0xffffffffu, (row << h.pbits) | (unsigned)i) == 0xffffffffu) break;
+            slot = (slot + 1) & (unsigned)(nslots - 1);
+        }
+    }
+    __syncthreads();
+    return h;
+}
+
+/* Position of `row` in I, or -1. */
+__device__ __forceinline__ int row_position(const RowHash& h, const int* I, int m, int row)
+{
+    if (h.tab) {
+        unsigned slot = ((unsigned)row * 0x9e3779b1u) >> (32 - h.bits);
+        const unsigned mask = (1u << h.bits) - 1u;
+        for (int probe = 0; probe <= (int)mask; ++probe) {
+            const unsigned e = h.tab[slot];
+            if (e == 0xffffffffu) return -1;
+            if ((e >> h.pbits) == (unsigned)row) return (int)(e & ((1u << h.pbits) - 1u));
+            slot = (slot + 1) & mask;
+        }
+    }
+    const int p = lower_bound_i(I, m, row);
+    return (p < m && I[p] == row) ? p : -1;
+}
+
 /* Augment_Sparsity (src/Spai_Sub.imp:1010-1116) with Union_Nl_Sets and
  * Compute_Rhos.  Returns the new |J|, -1 when J~ is empty, or kOverflow.
  * Without kTies a tie of scores across the cut also returns kOverflow: the
@@ -1342,6 +1401,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         return -1;
     }
     /* rho_j = sqrt(||r||^2 - (r^T a_j)^2 / ||a_j||^2), numerator over I only */
+    const RowHash h = row_hash_build<G>(s, L, m, a.A.nrows);
     bool rho_near0 = false;  // mixed path: rho^2 within rounding of 0 (its sign decides NaN or not)
     for (int t = lane; t < c; t += G) {
         int j = s.jt[t];
@@ -1359,8 +1419,8 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (e + q < e1) {
-                    int p = lower_bound_i(s.I, m, rows[q]);
-                    if (p < m && s.I[p] == rows[q]) num = __dadd_rn(num, __dmul_rn(s.r[p], vals[q]));
+                    const int p = row_position(h, s.I, m, rows[q]);
+                    if (p >= 0) num = __dadd_rn(num, __dmul_rn(s.r[p], vals[q]));
                 }
             }
         }
@@ -1374,17 +1434,39 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
     __syncthreads();
     if (kMixed && Gr::ballot(rho_near0) != 0ull) return kFallback;
     acc4 = Gr::sum_fixed(acc4);
-    double mean = 0.0;
-    if (lane == 0) {
-        for (int t = 0; t < c; ++t) mean = __dadd_rn(mean, s.rho[t]);
-        if (a.use_mean) mean = __ddiv_rn(mean, (double)c);
-        if (wk) {
-            wk[3] += acc3;
-            wk[4] += acc4;
-            wk[5] += (double)c;
-        }
+    if (wk && lane == 0) {
+        wk[3] += acc3;
+        wk[4] += acc4;
+        wk[5] += (double)c;
     }
-    mean = Gr::bcast0(mean);
+    /* The threshold is the reference's sequential sum of the scores (over
+     * the candidates in increasing j), divided by c with -um 1.  A parallel
+     * sum differs from it by at most 2 gamma_{c-1} sum|rho| (both orders are
+     * within gamma_{c-1} sum|rho| of the exact sum; rho >= 0), and the
+     * division adds at most one rounding to each: with
+     * band = 4 (c + 1) u threshold, no score outside the band around the
+     * parallel threshold compares differently with the sequential one.  The
+     * sequential sum runs only when some score lies inside the band (or in
+     * builds with counters, whose near-margin count is exact). */
+    double part = 0.0;
+    for (int t = lane; t < c; t += G) part += s.rho[t];
+    double mean = Gr::sum_fixed(part);
+    if (a.use_mean) mean = __ddiv_rn(mean, (double)c);
+    bool exact = wk != nullptr || !isfinite(mean);
+    if (!exact) {
+        const double band = 4.0 * (double)(c + 1) * 0x1p-53 * mean;
+        bool inside = false;
+        for (int t = lane; t < c; t += G) inside |= fabs(s.rho[t] - mean) <= band;
+        exact = Gr::ballot(inside) != 0ull;
+    }
+    if (exact) {
+        double seq = 0.0;
+        if (lane == 0) {
+            for (int t = 0; t < c; ++t) seq = __dadd_rn(seq, s.rho[t]);
+            if (a.use_mean) seq = __ddiv_rn(seq, (double)c);
+        }
+        mean = Gr::bcast0(seq);
+    }
     if (isnan(mean)) {
         if (kMixed) return kFallback;
         *st |= COL_NAN_RHO;
@@ -2021,6 +2103,36 @@ __device__ __forceinline__ int ls_solve_cached(const BuildArgs& a, const Scratch
     return st_solve;
 }
 
+/* A column that outgrows the launch's layout during `step` leaves the state
+ * of that step's start for the next launch: J in its output slot, |J| in
+ * m_len, the status bits, nsolve = -(step + 2) as the mark, and in builds
+ * with counters the work counters in their row and the near-margin counts
+ * packed into res (each below 2^20; read back exactly).  Every value the
+ * step computes depends only on these, so the resumed column is bitwise the
+ * column built in one launch.  Without save_state (a mixed launch whose
+ * overflow goes on in FP64) the mark is 0: the next launch starts afresh. */
+template <int G>
+__device__ __forceinline__ void save_column_state(const BuildArgs& a, const WaveScratch& s, int col, int step, int n,
+                                                  int st, const int (&near)[3], double wk, bool counters)
+{
+    const int lane = Group<G>::lane();
+    const size_t o = (size_t)col * a.cap;
+    if (!a.save_state) {
+        if (lane == 0) a.nsolve[col] = 0;
+        __syncthreads();
+        return;
+    }
+    for (int i = lane; i < n; i += G) a.m_idx[o + i] = s.J[i];
+    if (counters && lane < kNumWork) a.work[(size_t)col * kNumWork + lane] = wk;
+    if (lane == 0) {
+        a.m_len[col] = n;
+        a.status[col] = st;
+        a.nsolve[col] = -(step + 2);
+        if (counters) a.res[col] = near[0] + near[1] * 0x1p20 + near[2] * 0x1p40;
+    }
+    __syncthreads();
+}
+
 template <int G, bool kMixed, int NMAX, bool kTies>
 __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int col)
@@ -2028,17 +2140,56 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
     using Gr = Group<G>;
     const int lane = Gr::lane();
     double* wk = a.work ? s.wsum : nullptr;  // summed in LDS, stored with the column
-    int p0 = a.p_ptr[col], n = a.p_ptr[col + 1] - p0;
-    if (n > L.capJ) return kMoveOn;
-    if (wk && lane < kNumWork) wk[lane] = 0.0;
-    for (int i = lane; i < n; i += G) s.J[i] = a.p_ind[p0 + i];
-    __syncthreads();
-    int st = 0, solves = 0;
+    int st = 0, step0 = -1, n;
     int near[3] = {0, 0, 0};  // near-margin decisions (counters builds, FP64)
+    const size_t o = (size_t)col * a.cap;
+    const int mark = a.nsolve[col];
+    const int* jsrc;
+    if (mark < 0) {
+        /* resume the state an earlier launch of this precision left at the
+         * start of step -mark - 2 (see save_column_state) */
+        step0 = -mark - 2;
+        n = a.m_len[col];
+        st = a.status[col];
+        jsrc = a.m_idx + o;
+    } else {
+        const int p0 = a.p_ptr[col];
+        n = a.p_ptr[col + 1] - p0;
+        jsrc = a.p_ind + p0;
+    }
+    if (n > L.capJ) {
+        /* a saved state stays for the next launch when that one resumes it */
+        if (mark < 0 && !a.save_state && lane == 0) a.nsolve[col] = 0;
+        return kMoveOn;
+    }
+    if (wk) {
+        if (lane < kNumWork) wk[lane] = mark < 0 ? a.work[(size_t)col * kNumWork + lane] : 0.0;
+        if (mark < 0) {
+            const double pk = a.res[col];
+            near[2] = (int)(pk * 0x1p-40);
+            near[1] = (int)((pk - near[2] * 0x1p40) * 0x1p-20);
+            near[0] = (int)(pk - near[2] * 0x1p40 - near[1] * 0x1p20);
+        }
+    }
+    for (int i = lane; i < n; i += G) s.J[i] = jsrc[i];
+    __syncthreads();
+    int solves = step0 + 1;
     double rn = 1.0e50;
-    for (int step = -1; step < a.ns; ++step) {
+    /* the state at the start of the current step, left for the next launch
+     * if the column outgrows this one's layout during the step */
+    int moved = -2, st0 = st, near0[3] = {near[0], near[1], near[2]};
+    double wk0 = 0.0;
+    for (int step = step0; step < a.ns; ++step) {
+        st0 = st;
+        near0[0] = near[0];
+        near0[1] = near[1];
+        near0[2] = near[2];
+        if (wk && lane < kNumWork) wk0 = wk[lane];
         int m = build_I<G>(a, L, s, n);
-        if (m == kOverflow) return kMoveOn;
+        if (m == kOverflow) {
+            moved = step;
+            break;
+        }
         gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n);
         st |= make_rhs<G>(a, s, m, col);
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
@@ -2062,12 +2213,18 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         if (!kMixed && wk) near[2] += fabs(rn - a.epsilon) <= kNearMargin * a.epsilon;
         if (rn <= a.epsilon || step == a.ns - 1) break;
         int nn = augment<G, kMixed, kTies>(a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
-        if (nn == kOverflow) return kMoveOn;
+        if (nn == kOverflow) {
+            moved = step;  // J is unchanged: augment merges only once every check has passed
+            break;
+        }
         if (nn == kFallback) return kRedoFp64;
         if (nn < 0) break;
         n = nn;
     }
-    size_t o = (size_t)col * a.cap;
+    if (moved > -2) {
+        if (kMixed || a.save_state) save_column_state<G>(a, s, col, moved, n, st0, near0, wk0, wk != nullptr);
+        return kMoveOn;
+    }
     for (int i = lane; i < n; i += G) {
         a.m_idx[o + i] = s.J[i];
         a.m_val[o + i] = s.b[i];
@@ -2102,9 +2259,14 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
  * would admit more than the registers otherwise do: 5 for the FP64 8-wide
  * 32-lane kernel, 4 for its other modes and the generic (HBM working set)
  * kernels, 3 for the FP64 64-lane register kernel (each measured faster
- * despite the spill; the mixed 8-wide kernel is slower at 5). */
+ * despite the spill; the mixed 8-wide kernel is slower at 5), and 3 for the
+ * 16-lane and the 16-wide 32-lane builds, whose bins LDS holds at 10-12
+ * waves per CU. */
 template <int G, bool kLds, int kMode, int NMAX>
-__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4) : ((G == 64 && NMAX == 32 && kMode == 0) ? 3 : (!kLds ? 4 : 1))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
+__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)
+                                : ((G == 64 && NMAX == 32 && kMode == 0) ? 3
+                                   : ((kLds && kMode < 2 && (G == 16 || (G == 32 && NMAX == 16))) ? 3
+                                      : (!kLds ? 4 : 1)))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
 {
     extern __shared__ __align__(16) char smem[];
@@ -2121,6 +2283,15 @@ __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4) :
                       : a.scratch + ((size_t)blockIdx.x * NG + g) * L.per_wave;
     char* tie = L.tie_in_main ? base : a.tie_scratch + ((size_t)blockIdx.x * NG + g) * L.tie_per_group;
     WaveScratch s = wave_scratch(base, tie, L);
+    if (kLds) {
+        /* the hash of the scores lives in b0: pow2 <= 2 capB slots of 4 bytes in capB doubles */
+        const int capB = L.capI > L.capJ ? L.capI : L.capJ;
+        s.htab = (unsigned*)s.b0;
+        s.hslots = 1 << (31 - __clz(2 * capB));
+    } else {
+        s.htab = (unsigned*)smem;
+        s.hslots = kGenericHashSlots;
+    }
     /* columns per chunk: one per group, `rounds` rounds.  Every wave starts on
      * the chunk of its block index; when the queue fits in one chunk per wave
      * (at most L.max_rounds columns per group) that static split is all, else
@@ -2168,6 +2339,8 @@ __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4) :
             }
             __syncthreads();
             const bool leader = Group<G>::lane() == 0;
+            /* a column the mixed path declines starts afresh in FP64 */
+            if (kMode == 1 && leader && outcome == kRedoFp64) a.nsolve[col] = 0;
             wave_append(leader && outcome == kMoveOn, col, a.overflow, a.n_overflow);
             if (kMode == 1 || kMode == 3) wave_append(leader && outcome == kRedoFp64, col, a.fallback, a.n_fallback);
         }
@@ -2220,6 +2393,7 @@ __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, B
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     int bin = nbins;  // generic
     if (col < n) {
+        if (a.nsolve) a.nsolve[col] = 0;  // no saved state: the column starts from its start pattern
         int p0 = a.p_ptr[col], nj = a.p_ptr[col + 1] - p0;
         long raw = 0;
         for (int e = p0; e < p0 + nj; ++e) {
@@ -2360,14 +2534,15 @@ void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream)
 void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, int mode, const int* queue,
                          const int* nqueue, int* head, int nwaves, hipStream_t stream)
 {
+    const size_t lds = kGenericHashSlots * sizeof(unsigned);  // the score hash; the working set is in HBM
     if (mode == 2)
-        hipLaunchKernelGGL((spai_column_kernel<64, false, 2, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 2, 0>), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue,
                            nqueue, head);
     else if (mode == 4)
-        hipLaunchKernelGGL((spai_column_kernel<64, false, 4, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 4, 0>), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue,
                            nqueue, head);
     else
-        hipLaunchKernelGGL((spai_column_kernel<64, false, 0, 0>), dim3(nwaves), dim3(kWave), 0, stream, a, L, queue,
+        hipLaunchKernelGGL((spai_column_kernel<64, false, 0, 0>), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue,
                            nqueue, head);
 }
 

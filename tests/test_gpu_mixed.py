@@ -94,3 +94,35 @@ def test_mixed_repeat_routes_fp64_columns_alike(oracle, gpu):
     assert np.array_equal(info1["res_norm"].view(np.int64), info2["res_norm"].view(np.int64))
     assert np.array_equal(info1["nsolve"], info2["nsolve"]) and np.array_equal(info1["status"], info2["status"])
 
+
+
+def test_mixed_full_c4(oracle, gpu):
+    """MSPAI probing on the full 3D Laplacian 64^3 (n = 262,144), every
+    column against the oracle: identical pattern, <= 1e-9 per column."""
+    prob = gen.config4()
+    print(f"[mixed full {prob.name}] n={prob.n}", flush=True)
+    M, info, stats = gpu.spai(prob, precision=1)
+    worst, fbfrac = check_mixed(oracle, prob, M, info)
+    print(f"   worst column rel err {worst:.2e}, fallback fraction {fbfrac:.4f}", flush=True)
+
+
+def test_mixed_full_c5_sampled_columns(oracle, gpu):
+    """n = 2,097,152 (static P(A^2)): the mixed build of every column, a
+    stratified sample of 8,192 columns re-solved by the oracle: the same
+    pattern, columns within 1e-9, residual norms within 1e-9."""
+    prob = gen.config5()
+    print(f"[mixed full {prob.name}] n={prob.n}", flush=True)
+    M, info, stats = gpu.spai(prob, precision=1)
+    assert np.array_equal(M.indptr, prob.P.indptr) and np.array_equal(M.indices, prob.P.indices)
+    cols = np.sort(np.random.Generator(np.random.PCG64(3)).choice(prob.n, size=8192, replace=False)).astype(np.int32)
+    ref = oracle.spai(prob, threads=THREADS, cols=cols)
+    worst = 0.0
+    for t, k in enumerate(cols):
+        a, b = M.indptr[k], M.indptr[k + 1]
+        rv = ref["val"][t, :ref["len"][t]]
+        assert np.array_equal(M.indices[a:b], ref["idx"][t, :ref["len"][t]])
+        worst = max(worst, float(np.linalg.norm(M.data[a:b] - rv) / max(np.linalg.norm(rv), 1e-300)))
+    rr = np.abs(info["res_norm"][cols] - ref["res_norm"]) / np.maximum(ref["res_norm"], 1e-300)
+    print(f"   worst sampled column rel err {worst:.2e}, residual {rr.max():.2e}, "
+          f"fallbacks {stats['n_fallback']}", flush=True)
+    assert worst <= TOL and rr.max() <= TOL

@@ -369,7 +369,9 @@ def test_launch_stats_and_near_margins(oracle, gpu, name):
           flush=True)
     assert sum(x["columns"] for x in ls) == prob.n
     assert sum(x["nnz"] for x in ls) == ref["len"].sum()
-    assert np.array_equal(np.sum([x["work"] for x in ls], axis=0), tot)
+    per_launch = np.sum([x["work"] for x in ls], axis=0)
+    assert np.array_equal(per_launch[1:], tot[1:])  # integer counters: exact in any order
+    assert np.allclose(per_launch[0], tot[0], rtol=1e-12, atol=0)  # QR flop 2mn^2 - 2n^3/3: fractional
     assert np.array_equal(np.sum([x["near"] for x in ls], axis=0), ref["near"].sum(axis=0))
     assert all(x["ms"] >= 0 for x in ls)
     assert [x["name"] for x in ls_off] == [x["name"] for x in ls]

@@ -2195,14 +2195,17 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {
             if (ls_solve_mixed<G, NMAX>(a, L, s, m, n) == kFallback) return kRedoFp64;
-        } else if (a.cache_slots > 0) {
-            st |= ls_solve_cached<G>(a, L, s, m, n);
         } else if (NMAX > 0 && m <= G && n <= NMAX) {
+            /* the register QR costs less than a cache lookup (which reads the
+             * stored block and factor back from HBM): -cs serves the other
+             * solves; both give the same bits */
             if constexpr (G == kWave && NMAX == 32)
                 st |= ls_solve_rows64<false>(L, s, m, n);
             else
                 st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n);
             dense = true;
+        } else if (a.cache_slots > 0) {
+            st |= ls_solve_cached<G>(a, L, s, m, n);
         } else {
             st |= ls_solve<G>(L, s, m, n);
         }

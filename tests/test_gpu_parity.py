@@ -247,18 +247,23 @@ def test_qr_update_full_c3(oracle, gpu):
                                   lambda: gen.config3(n=16384, bw=800), lambda: csparse_problem("fs_183_1")])
 def test_cache_is_bitwise_neutral(oracle, gpu, make):
     """-cs N: cached factors/solutions reproduce the uncached build bit for bit,
-    and the structured grids hit the cache."""
+    and the structured grids hit the cache.  The size bins solve register
+    columns without the cache (the register QR costs less than a lookup), so
+    the cache is also run with the generic kernel alone (binning 0), where
+    every solve goes through it."""
     prob = make()
     ref = oracle.spai(prob, threads=THREADS)
-    M, info, stats = gpu.spai(prob, cache=4096)
-    assert_bitwise(oracle, prob, ref, M, info)
-    print(f"[cache {prob.name}] hits {stats['n_cache_hits']} of {int(info['nsolve'].sum())} solves", flush=True)
-    # a hit needs an identical block published by another group before the
-    # lookup, so the count depends on timing: a small static problem runs all
-    # its columns at once and may hit nothing, while the adaptive 2D grid
-    # (second solves of 65,536 columns) must reuse most of its blocks
-    if prob.name == "convdiff2d_adaptive":
-        assert stats["n_cache_hits"] > 0.5 * info["nsolve"].sum()
+    for binning in (1, 0):
+        M, info, stats = gpu.spai(prob, cache=4096, binning=binning)
+        assert_bitwise(oracle, prob, ref, M, info)
+        print(f"[cache {prob.name} binning={binning}] hits {stats['n_cache_hits']} of "
+              f"{int(info['nsolve'].sum())} solves", flush=True)
+        # a hit needs an identical block published by another group before the
+        # lookup, so the count depends on timing: a small static problem runs all
+        # its columns at once and may hit nothing, while the adaptive 2D grid
+        # (second solves of 65,536 columns) must reuse most of its blocks
+        if prob.name == "convdiff2d_adaptive" and binning == 0:
+            assert stats["n_cache_hits"] > 0.5 * info["nsolve"].sum()
 
 
 def test_autotuned_bins_bitwise(oracle, gpu):

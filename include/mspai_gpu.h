@@ -112,6 +112,29 @@ int mspai_set_target(mspai_ctx* ctx, const int* colptr, const int* rowind, const
  * device until downloaded.  `stats` may be NULL. */
 int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* stats);
 
+/*
+ * mspai_build without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The inputs are those already in the context (matrix, start pattern --
+ * set beforehand, since setting one allocates --, target); M is written to
+ * the caller's device arrays d_colptr[ncols + 1], d_rowind[max_nnz] and
+ * d_val[max_nnz], which must hold the worst case max_nnz >= ncols * (max |J0|
+ * + maxnew * max_impr_steps), the capacity Init_Preconditioner reserves.
+ * Everything else the build needs (queues, counters, output slots, per-column
+ * results, working sets) lives in a static device array of
+ * MSPAI_NOALLOC_SCRATCH_MIB MiB that each device holds until the process
+ * ends; a problem that needs more fails with MSPAI_ERR_CAPACITY before any
+ * launch (about 130 MiB for the random band n = 262,144 with 8 steps of 4).
+ * FP64 or mixed, any binning; not with the cache (-cs), block SPAI or work
+ * counters (MSPAI_ERR_UNSUPPORTED).  M is not kept in the context
+ * (mspai_download_* and mspai_reuse_pattern do not see it).  Calls that
+ * overlap on one GPU share the array and race.  Returns when M is written.
+ */
+#define MSPAI_NOALLOC_SCRATCH_MIB 256
+int mspai_build_noalloc(mspai_ctx* ctx, const mspai_opts* opts, int* d_colptr, int* d_rowind, double* d_val,
+                        long long max_nnz, mspai_stats* stats);
+
 /* nnz of the last built M. */
 int mspai_result_nnz(const mspai_ctx* ctx, long long* nnz);
 

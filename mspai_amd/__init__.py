@@ -22,7 +22,7 @@ COL_SINGULAR, COL_UNDERDET, COL_NAN_RHO, COL_FALLBACK, COL_K_NOT_IN_I = 1, 2, 4,
 EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspai_last_error",
             "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_reuse_pattern",
             "mspai_set_target",
-            "mspai_build", "mspai_result_nnz", "mspai_download_csc",
+            "mspai_build", "mspai_build_noalloc", "mspai_result_nnz", "mspai_download_csc",
             "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts", "mspai_launch_stats",
             "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
             "mspai_backward", "mspai_autotune", "mspai_apply",
@@ -184,6 +184,19 @@ class Builder:
         return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz,
                 "ncols": st.ncols, "n_fallback": st.n_fallback, "n_overflow": st.n_overflow,
                 "n_cache_hits": st.n_cache_hits, "first_bin_ms": st.first_bin_ms}
+
+    def build_noalloc(self, colptr_ptr: int, rowind_ptr: int, val_ptr: int, max_nnz: int, epsilon=0.4, maxnew=5,
+                      max_impr_steps=5, use_mean=1, qr=0, precision=0, binning=1):
+        """Build M into caller-owned device arrays (colptr: ncols + 1 int32;
+        rowind/val: max_nnz int32/float64, max_nnz >= ncols * (max |J0| +
+        maxnew * max_impr_steps)) without any device allocation: the
+        library's static scratch array holds the rest."""
+        o = Opts(epsilon, maxnew, max_impr_steps, use_mean, qr, 0, precision, binning, 0, 0)
+        st = Stats()
+        self._chk(self.lib.mspai_build_noalloc(self.h, C.byref(o), C.c_void_p(colptr_ptr), C.c_void_p(rowind_ptr),
+                                               C.c_void_p(val_ptr), C.c_longlong(max_nnz), C.byref(st)),
+                  "mspai_build_noalloc")
+        return {"build_ms": st.build_ms, "solve_ms": st.solve_ms, "nnz": st.nnz, "n_fallback": st.n_fallback}
 
     # ---- device-pointer entry points (addresses of device buffers, e.g. tensor storage)
     def update_values_device(self, ptr: int):

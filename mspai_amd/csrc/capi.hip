@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,20 @@ using namespace mspai;
 
 constexpr int kMaxBins = 4;
 constexpr int kMaxLaunch = 2 * kMaxBins + 2;  // column-kernel launches of one build: mixed bins, FP64 bins, generic
+
+/* Device scratch of mspai_build_noalloc (queues, counters, output slots,
+ * per-column results, tie buffers, the scan workspace and the generic
+ * kernel's working sets of one build), a static array so that the call
+ * allocates nothing.  Each device the library runs on holds its own copy
+ * until the process ends.  In host code `scratch` only names the symbol:
+ * its device address comes from hipGetSymbolAddress, once per device (at
+ * mspai_ctx_create), and that address is what the kernels receive.  Calls
+ * on one device share the array: calls that overlap on one GPU race. */
+constexpr size_t kNoAllocBytes = (size_t)MSPAI_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static char* g_noalloc_ptr[kMaxDevices] = {};
 
 namespace {
 
@@ -146,6 +161,11 @@ struct mspai_ctx {
     char lname[kMaxLaunch][48] = {};
     double* d_lwork = nullptr;  // kMaxLaunch x kLaunchStat
     bool lwork_valid = false;
+    /* mspai_build_noalloc: the build's buffers point into the static scratch
+     * array for the duration of the call, and every path that would
+     * allocate fails with MSPAI_ERR_CAPACITY instead */
+    bool no_alloc = false;
+    char* d_noalloc = nullptr;  // this device's address of the static scratch array
     /* pattern->R cache */
     int cache_slots = 0, cache_capI = 0, cache_capJ = 0;
     int* d_cache_state = nullptr;
@@ -200,6 +220,7 @@ static int fail(mspai_ctx* ctx, int code, const std::string& msg)
 template <class T>
 static int upload(mspai_ctx* ctx, T*& d, const T* h, size_t n)
 {
+    if (ctx->no_alloc) return fail(ctx, MSPAI_ERR_CAPACITY, "this call allocates no device memory");
     dfree(d);
     if (n == 0) n = 1;
     HIPCHK(hipMalloc((void**)&d, n * sizeof(T)));
@@ -279,6 +300,18 @@ extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
             delete ctx;
             return MSPAI_ERR_HIP;
         }
+    if (device < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_ptr[device]) {
+            void* p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+                delete ctx;
+                return MSPAI_ERR_HIP;
+            }
+            g_noalloc_ptr[device] = (char*)p;
+        }
+        ctx->d_noalloc = g_noalloc_ptr[device];
+    }
     if (hipMalloc((void**)&ctx->d_counter, 64 * sizeof(int)) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_counter, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&ctx->d_wsum, kNumWork * sizeof(double)) != hipSuccess ||
@@ -579,6 +612,7 @@ static int ensure_outputs(mspai_ctx* ctx, int cap)
 {
     size_t n = ctx->ncols;
     if (ctx->cap == cap && ctx->out_cols == n && ctx->d_m_len) return MSPAI_OK;
+    if (ctx->no_alloc) return fail(ctx, MSPAI_ERR_CAPACITY, "outputs: this call allocates no device memory");
     dfree(ctx->d_m_len);
     dfree(ctx->d_m_idx);
     dfree(ctx->d_m_val);
@@ -790,6 +824,7 @@ static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, b
 static int ensure_scratch(mspai_ctx* ctx, size_t need)
 {
     if (need <= ctx->scratch_bytes) return MSPAI_OK;
+    if (ctx->no_alloc) return fail(ctx, MSPAI_ERR_CAPACITY, "working sets: this call allocates no device memory");
     dfree(ctx->d_scratch);
     HIPCHK(hipMalloc((void**)&ctx->d_scratch, need));
     ctx->scratch_bytes = need;
@@ -835,6 +870,17 @@ static int end_launch(mspai_ctx* ctx)
     return MSPAI_OK;
 }
 
+/* HBM tie-path buffers of the LDS bins: one region per resident group. */
+static size_t tie_bytes_needed(const mspai_ctx* ctx, const std::vector<Bin>& bins, bool mixed, int fp64_mode)
+{
+    size_t need = 0;
+    for (const Bin& b : bins)
+        need = std::max(need, (size_t)ctx->num_cu *
+                                  std::max(waves_per_cu(b, fp64_mode), mixed ? waves_per_cu(b, fp64_mode == 2 ? 3 : 1) : 0) *
+                                  (64 / b.group) * b.L.tie_per_group);
+    return need;
+}
+
 /* Most FP64 columns of a mixed build that the generic kernel takes alone. */
 constexpr int kDirectFp64 = 4096;
 
@@ -863,19 +909,15 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     int* fq = ctx->d_queues;
     int* mq = ctx->d_queues + (size_t)(kMaxBins + 1) * n;
     int rc;
-    HIPCHK(hipMemsetAsync(ctx->d_counter, 0, 64 * sizeof(int), ctx->stream));
+    launch_fill_int(ctx->d_counter, 64, 0, ctx->stream);  // a kernel, not a memset node
+    HIPCHK(hipGetLastError());
     ctx->nlaunch = 0;
     ctx->lwork_valid = a.work != nullptr;
     if (a.work) HIPCHK(hipMemsetAsync(ctx->d_lwork, 0, kMaxLaunch * kLaunchStat * sizeof(double), ctx->stream));
     a.status_or = 0;
-    /* HBM tie-path buffers of the LDS bins: one region per resident group */
-    size_t tie_need = 0;
-    for (const Bin& b : bins)
-        tie_need = std::max(tie_need, (size_t)ctx->num_cu *
-                                          std::max(waves_per_cu(b, fp64_mode),
-                                                   mixed ? waves_per_cu(b, fp64_mode == 2 ? 3 : 1) : 0) *
-                                          (64 / b.group) * b.L.tie_per_group);
+    const size_t tie_need = tie_bytes_needed(ctx, bins, mixed, fp64_mode);
     if (tie_need > ctx->tie_bytes) {
+        if (ctx->no_alloc) return fail(ctx, MSPAI_ERR_CAPACITY, "tie buffers: this call allocates no device memory");
         dfree(ctx->d_tie);
         HIPCHK(hipMalloc((void**)&ctx->d_tie, tie_need));
         ctx->tie_bytes = tie_need;
@@ -988,6 +1030,7 @@ static int read_chain_counts(mspai_ctx* ctx, int nbins, bool mixed, mspai_stats*
  * at most 2^18; each slot holds up to 32 x 16 blocks. */
 static int ensure_cache(mspai_ctx* ctx, int entries, int capI, int capJ)
 {
+    if (ctx->no_alloc) return fail(ctx, MSPAI_ERR_UNSUPPORTED, "the cache (-cs) allocates device memory");
     int slots = 1;
     while (slots < entries && slots < (1 << 18)) slots <<= 1;
     if (slots == ctx->cache_slots && capI == ctx->cache_capI && capJ == ctx->cache_capJ) return MSPAI_OK;
@@ -1014,7 +1057,8 @@ static int ensure_cache(mspai_ctx* ctx, int entries, int capI, int capJ)
 static int prepare_generic(mspai_ctx* ctx, const ScratchLayout& G, int* gwaves)
 {
     int w = std::max(1, ctx->num_cu) * 16;  // 4 waves per SIMD, the generic kernels' register budget
-    const size_t budget = (size_t)16 << 30;
+    /* without allocation: as many waves as the given working-set region holds */
+    const size_t budget = ctx->no_alloc ? ctx->scratch_bytes : (size_t)16 << 30;
     while (w > 1 && (size_t)w * G.per_wave > budget) w /= 2;
     if ((size_t)w * G.per_wave > budget) return fail(ctx, MSPAI_ERR_CAPACITY, "per-column working set too large");
     *gwaves = w;
@@ -1143,6 +1187,120 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         stats->n_cache_hits = a.cache_slots > 0 ? ctx->h_counter[48] : 0;
     }
     return MSPAI_OK;
+}
+
+extern "C" int mspai_build_noalloc(mspai_ctx* ctx, const mspai_opts* opts, int* d_colptr, int* d_rowind,
+                                   double* d_val, long long max_nnz, mspai_stats* stats)
+{
+    if (!ctx || !opts || !d_colptr || !d_rowind || !d_val) return MSPAI_ERR_ARG;
+    if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "build_noalloc: upload the matrix first");
+    if (!ctx->d_noalloc) return fail(ctx, MSPAI_ERR_STATE, "build_noalloc: no scratch address for this device");
+    if (ctx->block_bs != 1 || opts->cache > 0 || opts->counters)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED, "build_noalloc: not with block SPAI, the cache (-cs) or work counters");
+    if (!ctx->d_p_ptr)
+        return fail(ctx, MSPAI_ERR_STATE, "build_noalloc: set the start pattern first (mspai_set_pattern allocates)");
+    int rc;
+    if ((rc = check_opts(ctx, opts))) return rc;
+    if ((rc = check_shapes(ctx))) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = ctx->ncols;
+    const int cap = ctx->plen_max + opts->maxnew * opts->max_impr_steps;
+    if (cap <= 0) return fail(ctx, MSPAI_ERR_ARG, "build: empty start pattern and no augmentation");
+    if (max_nnz < (long long)(n * cap))
+        return fail(ctx, MSPAI_ERR_CAPACITY, "build_noalloc: max_nnz must be at least ncols * (max |J0| + mn * ns) = " +
+                                                 std::to_string(n * cap));
+    /* carve the static array: the build's fixed buffers first, then the
+     * tie buffers of the size bins, the scan workspace, and what is left
+     * for the generic kernel's working sets */
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = off;
+        off = align_up(off + bytes, 256);
+        return ctx->d_noalloc + at;
+    };
+    const size_t fixed = align_up(64 * sizeof(int), 256) + align_up((2 * kMaxBins + 1) * n * sizeof(int), 256) +
+                         align_up(n * sizeof(int), 256) + align_up(n * cap * sizeof(int), 256) +
+                         align_up(n * cap * sizeof(double), 256) + align_up(n * sizeof(double), 256) +
+                         2 * align_up(n * sizeof(int), 256);
+    const bool adaptive = opts->max_impr_steps > 0;
+    const bool qrup = opts->qr == 2 && adaptive && opts->maxnew > 0;
+    const int needs = (qrup ? kNeedQrUpdate : 0) | (opts->precision == 1 && !qrup ? kNeedMixed : 0);
+    const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive, needs);
+    std::vector<Bin> bins;
+    if (opts->binning) {
+        int t = (opts->binning == 2 && ctx->tuned_table >= 0) ? ctx->tuned_table : 0;
+        bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t], needs);
+    }
+    const bool mixed = opts->precision == 1 && !bins.empty() && !qrup;
+    const size_t tie = align_up(tie_bytes_needed(ctx, bins, mixed, qrup ? 4 : 0), 256);
+    size_t cub = 0;
+    hipcub::DeviceScan::InclusiveSum(nullptr, cub, (const int*)nullptr, (int*)nullptr, (int)n, ctx->stream);
+    cub = align_up(std::max<size_t>(cub, 256), 256);
+    if (fixed + tie + cub + G.per_wave > kNoAllocBytes)
+        return fail(ctx, MSPAI_ERR_CAPACITY, "build_noalloc: the problem needs more than the " +
+                                                 std::to_string(MSPAI_NOALLOC_SCRATCH_MIB) + " MiB scratch array");
+    /* the context's own buffers step aside for the call */
+    struct Saved {
+        int *counter, *queues, *m_len, *m_idx, *nsolve, *status, *Mptr, *Mind;
+        double *m_val, *res, *work, *Mval;
+        char *tie, *scr;
+        void* cub;
+        size_t tie_bytes, cub_bytes, scr_bytes, Mcap, out_cols;
+        int cap;
+    } sv{ctx->d_counter, ctx->d_queues, ctx->d_m_len, ctx->d_m_idx, ctx->d_nsolve, ctx->d_status, ctx->d_Mptr,
+         ctx->d_Mind, ctx->d_m_val, ctx->d_res, ctx->d_work, ctx->d_Mval, ctx->d_tie, ctx->d_scratch, ctx->d_cub,
+         ctx->tie_bytes, ctx->cub_bytes, ctx->scratch_bytes, ctx->Mcap, ctx->out_cols, ctx->cap};
+    ctx->d_counter = (int*)take(64 * sizeof(int));
+    ctx->d_queues = (int*)take((2 * kMaxBins + 1) * n * sizeof(int));
+    ctx->d_m_len = (int*)take(n * sizeof(int));
+    ctx->d_m_idx = (int*)take(n * cap * sizeof(int));
+    ctx->d_m_val = (double*)take(n * cap * sizeof(double));
+    ctx->d_res = (double*)take(n * sizeof(double));
+    ctx->d_nsolve = (int*)take(n * sizeof(int));
+    ctx->d_status = (int*)take(n * sizeof(int));
+    ctx->d_tie = take(tie);
+    ctx->tie_bytes = tie;
+    ctx->d_cub = take(cub);
+    ctx->cub_bytes = cub;
+    ctx->d_scratch = ctx->d_noalloc + off;
+    ctx->scratch_bytes = kNoAllocBytes - off;
+    ctx->d_work = nullptr;
+    ctx->d_Mptr = d_colptr;
+    ctx->d_Mind = d_rowind;
+    ctx->d_Mval = d_val;
+    ctx->Mcap = (size_t)max_nnz;
+    ctx->cap = cap;
+    ctx->out_cols = n;
+    ctx->no_alloc = true;
+    rc = mspai_build(ctx, opts, stats);
+    /* back to the context's buffers; M lives in the caller's arrays */
+    ctx->d_counter = sv.counter;
+    ctx->d_queues = sv.queues;
+    ctx->d_m_len = sv.m_len;
+    ctx->d_m_idx = sv.m_idx;
+    ctx->d_m_val = sv.m_val;
+    ctx->d_res = sv.res;
+    ctx->d_nsolve = sv.nsolve;
+    ctx->d_status = sv.status;
+    ctx->d_tie = sv.tie;
+    ctx->tie_bytes = sv.tie_bytes;
+    ctx->d_cub = sv.cub;
+    ctx->cub_bytes = sv.cub_bytes;
+    ctx->d_scratch = sv.scr;
+    ctx->scratch_bytes = sv.scr_bytes;
+    ctx->d_work = sv.work;
+    ctx->d_Mptr = sv.Mptr;
+    ctx->d_Mind = sv.Mind;
+    ctx->d_Mval = sv.Mval;
+    ctx->Mcap = sv.Mcap;
+    ctx->cap = sv.cap;
+    ctx->out_cols = sv.out_cols;
+    ctx->no_alloc = false;
+    ctx->nnzM = -1;
+    ctx->pattern_nnz = -1;
+    ctx->m_csr_ok = false;
+    ctx->lwork_valid = false;
+    return rc;
 }
 
 extern "C" int mspai_update_values_device(mspai_ctx* ctx, const double* d_val)

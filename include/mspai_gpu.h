@@ -183,14 +183,22 @@ int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const double* d_gM, d
  * and keeps the fastest for later builds with opts->binning == 2.  Results do
  * not depend on the table.  A new mspai_upload_csc clears the choice.
  */
-#define MSPAI_MAX_TUNE 12
+#define MSPAI_MAX_TUNE 16
 typedef struct {
     int ncandidates;
     int chosen;
     char chosen_name[32];
-    double candidate_ms[MSPAI_MAX_TUNE];  /* best column-kernel time per table */
+    double candidate_ms[MSPAI_MAX_TUNE];  /* best column-kernel time per table (-1: not tried) */
 } mspai_tune_report;
 int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeats, mspai_tune_report* report);
+
+/* Sets the bin table that builds of `precision` (0 FP64, 1 mixed) with
+ * opts->binning == 2 use, as the autotuner would (no counterpart in the
+ * reference: for tests and for callers that tune once and store the choice).
+ * `name` (32 bytes, may be NULL) receives the table's name, `ntables` (may be
+ * NULL) the number of tables; table < 0 only reports that number.  Mixed
+ * builds need a table with size bins (MSPAI_ERR_UNSUPPORTED otherwise). */
+int mspai_set_bin_table(mspai_ctx* ctx, int precision, int table, char* name, int* ntables);
 
 /* Sums over all columns of the 8 work counters of the last build (which
  * must have run with opts->counters set; else MSPAI_ERR_STATE):

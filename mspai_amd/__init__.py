@@ -25,7 +25,7 @@ EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspa
             "mspai_build", "mspai_build_noalloc", "mspai_result_nnz", "mspai_download_csc",
             "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts", "mspai_launch_stats",
             "mspai_update_values_device", "mspai_update_target_device", "mspai_result_values_device",
-            "mspai_backward", "mspai_autotune", "mspai_apply",
+            "mspai_backward", "mspai_autotune", "mspai_set_bin_table", "mspai_apply",
             "mspai_set_block_size", "mspai_block_info", "mspai_block_sizes"]
 
 
@@ -49,7 +49,7 @@ class Stats(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("ncandidates", C.c_int), ("chosen", C.c_int), ("chosen_name", C.c_char * 32),
-                ("candidate_ms", C.c_double * 12)]
+                ("candidate_ms", C.c_double * 16)]
 
 
 _lib = None
@@ -239,6 +239,15 @@ class Builder:
         self._chk(self.lib.mspai_autotune(self.h, C.byref(o), repeats, C.byref(r)), "mspai_autotune")
         return {"chosen": r.chosen, "name": r.chosen_name.decode(),
                 "candidate_ms": list(r.candidate_ms[:r.ncandidates])}
+
+    def set_bin_table(self, table, precision=0):
+        """Pin the bin table of builds with binning=2 for `precision`; returns
+        its name. table=-1 returns the number of tables instead."""
+        name = C.create_string_buffer(32)
+        cnt = C.c_int()
+        self._chk(self.lib.mspai_set_bin_table(self.h, precision, table, name, C.byref(cnt)),
+                  "mspai_set_bin_table")
+        return cnt.value if table < 0 else name.value.decode()
 
     def work_totals(self):
         w = np.zeros(NWORK, np.float64)

@@ -81,9 +81,11 @@ __global__ void work_sum_kernel(const double* work, int n, double* out)
 
 
 /* M's CSC from the per-column outputs: colptr[1..ncols] holds the inclusive
- * scan of the lengths; this kernel also sets colptr[0] = 0 and *nnz. */
+ * scan of the lengths; this kernel also sets colptr[0] = 0 and *nnz, and
+ * counts the columns the FP64 kernels built in a mixed build (*nfp64). */
 __global__ void scatter_columns_kernel(int ncols, int cap, const int* m_len, const int* m_idx,
-                                       const double* m_val, int* colptr, int* rowind, double* val, int* nnz)
+                                       const double* m_val, int* colptr, int* rowind, double* val, int* nnz,
+                                       const int* status, int* nfp64)
 {
     int col = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
@@ -91,6 +93,7 @@ __global__ void scatter_columns_kernel(int ncols, int cap, const int* m_len, con
     int len = m_len[col], o = col == 0 ? 0 : colptr[col];
     if (lane == 0 && col == 0) colptr[0] = 0;
     if (lane == 0 && col == ncols - 1) *nnz = o + len;
+    if (lane == 0 && nfp64 && (status[col] & COL_FALLBACK)) atomicAdd(nfp64, 1);
     size_t src = (size_t)col * cap;
     for (int i = lane; i < len; i += 64) {
         rowind[o + i] = m_idx[src + i];
@@ -105,6 +108,7 @@ struct mspai_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int num_cu = 0;
+    size_t lds_block = 64 * 1024;  // LDS one workgroup may hold (sharedMemPerBlock)
     // matrix C
     int nrows = 0, ncols = 0;
     long long nnz = 0;
@@ -174,7 +178,7 @@ struct mspai_ctx {
     int* d_cache_n = nullptr;
     int* d_cache_status = nullptr;
     double* d_cache_data = nullptr;
-    int tuned_table = -1;  // autotuner's choice for this matrix (binning == 2)
+    int tuned_table[2] = {-1, -1};  // autotuner's choice for this matrix (binning == 2), FP64 and mixed
     int last_table = 0;
     int last_nbins = 0;
     bool last_mixed = false;
@@ -286,6 +290,7 @@ extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
     mspai_ctx* ctx = new mspai_ctx();
     ctx->device = device;
     ctx->num_cu = prop.multiProcessorCount;
+    ctx->lds_block = std::min<size_t>(std::max<size_t>(prop.sharedMemPerBlock, 64 * 1024), 160 * 1024);
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
         delete ctx;
         return MSPAI_ERR_HIP;
@@ -439,7 +444,7 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     ctx->mixed_fp64_last = -1;
     ctx->a_csr_ok = ctx->m_csr_ok = ctx->ajn_ok = false;
     ctx->pattern_nnz = -1;
-    ctx->tuned_table = -1;
+    ctx->tuned_table[0] = ctx->tuned_table[1] = -1;
     if (ctx->block_bs != 1) {
         /* a new matrix leaves block mode; its block pattern no longer applies */
         block_free(ctx);
@@ -668,7 +673,7 @@ enum LayoutNeeds { kNeedMixed = 1, kNeedBackward = 2, kNeedQrUpdate = 4 };
  * tie_in_main, else in a separate per-group HBM region of tie_per_group
  * bytes (LDS bins: the path is rare). */
 static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capSort, int maxnew, size_t capPairs,
-                                 int needs, bool tie_in_main, int group = 0)
+                                 int needs, bool tie_in_main, int group = 0, size_t capSid = 0)
 {
     ScratchLayout L{};
     capI = std::max<size_t>(capI, 1);
@@ -688,18 +693,35 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     size_t capB = std::max<size_t>(capI, (size_t)capJ);
     L.J = take((size_t)capJ * 4 + 16);
     L.I = take(capI * 4);
-    L.sbuf = take(capSort * 4);
-    L.aqr = take(capI * capJ * 8);
+    if (needs & kNeedQrUpdate) {
+        /* the QR update keeps its factor in aqr across the steps */
+        L.sbuf = take(capSort * 4);
+        L.aqr = take(capI * capJ * 8);
+        L.jt = take(capU * 4);
+        L.rho = take(capU * 8);
+    } else {
+        /* A(I,J) lives from the gather to the residual, the raw row lists
+         * (sbuf) in Get_I_Set before the gather and in the augmentation
+         * after the residual, with the candidates and their scores (jt,
+         * rho): one region holds either */
+        const size_t sb = align_up(capSort * 4, 16), jb = align_up(capU * 4, 16);
+        const size_t at = take(std::max(capI * capJ * 8, sb + jb + capU * 8));
+        L.aqr = at;
+        L.sbuf = at;
+        L.jt = at + sb;
+        L.rho = at + sb + jb;
+    }
     L.b = take(capB * 8);
     L.b0 = take(capB * 8);
     L.r = take(capB * 8);
     L.tau = take((size_t)capJ * 8);
-    L.jt = take(capU * 4);
-    L.rho = take(capU * 8);
     L.sel = take((size_t)std::max(1, maxnew) * 4);
     L.segs = take(((size_t)capJ + 1) * 4);
     L.segoff = take(((size_t)capJ + 1) * 4);
-    L.sid = take(capSort * 4);
+    /* the flat entries of J's columns: at most |J| * max column nnz, and
+     * never more than the raw list they are sorted in */
+    L.capSid = (int)std::min<size_t>(capSort, capSid > 0 ? capSid : capSort);
+    L.sid = take((size_t)L.capSid * 4);
     L.wsum = take(8 * 8);
     /* register bins convert A(I,J) to FP32 as they load it: aq32 holds R only */
     const bool regbin = register_bin(group, (int)capI, capJ);
@@ -755,7 +777,7 @@ static ScratchLayout generic_layout(const mspai_ctx* ctx, int cap, int maxnew, b
     size_t capI = std::max<size_t>(std::min<size_t>(ctx->nrows, rawI), 1);
     size_t rawU = adaptive ? (capI + 1) * std::max(1, ctx->maxrownnz) : 1;
     size_t capU = adaptive ? std::min<size_t>(ctx->ncols, rawU) : 1;
-    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU, needs, true);
+    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU, needs, true, 0, rawI);
 }
 
 /* One size bin: a lane-group width, a resident-wave cap and the LDS layout
@@ -795,6 +817,29 @@ static const BinTable kTables[] = {
     {"g16-g64/occ8", 2, {{16, 8, 16, 64, 128, 8}, {64, 40, 128, 512, 1024, 0}}},
     {"g32-g64-g64/occ8", 3, {{32, 16, 32, 128, 256, 8}, {64, 32, 64, 256, 512, 0}, {64, 40, 128, 512, 1024, 0}}},
     {"g16-g64/occ10", 2, {{16, 8, 16, 64, 128, 10}, {64, 40, 128, 512, 1024, 0}}},
+    /* long candidate lists: a last bin of |I| <= 384 and raw lists to 8192
+     * (about 117 KB of LDS per wave at |J| <= 33, one wave per CU) keeps the
+     * widest columns of the random band out of the HBM working set */
+    {"g16-g64-g64-g64x384", 4,
+     {{16, 8, 16, 64, 128, 0}, {64, 32, 64, 512, 1024, 0}, {64, 40, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
+    {"g16-g64-g64x192-g64x384", 4,
+     {{16, 8, 16, 64, 128, 0}, {64, 32, 64, 512, 1024, 0}, {64, 40, 192, 2048, 4096, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
+    {"g32-g64-g64-g64x384", 4,
+     {{32, 16, 32, 256, 512, 0}, {64, 32, 64, 512, 1024, 0}, {64, 40, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
+    {"g64-g64-g64x384", 3,
+     {{64, 32, 64, 512, 1024, 0}, {64, 40, 128, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    /* no size bins: every column in the generic kernel */
+    {"generic", 0, {}},
+    /* |J| <= 16 in the bulk bin: half the A(I,J) region, about 14 KB of LDS
+     * per wave, 11 resident waves per CU instead of 7 */
+    {"g64x16-g64-g64x384", 3,
+     {{64, 16, 64, 512, 1024, 0}, {64, 40, 128, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    {"g16-g64x16-g64-g64x384", 4,
+     {{16, 8, 16, 64, 128, 0}, {64, 16, 64, 512, 1024, 0}, {64, 40, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 
@@ -804,7 +849,6 @@ constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, bool adaptive,
                                     const ScratchLayout& G, const BinTable& table, int needs)
 {
-    (void)ctx;
     std::vector<Bin> out;
     for (int t = 0; t < table.count; ++t) {
         const BinEdge& e = table.e[t];
@@ -813,12 +857,21 @@ static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, b
         size_t capU = adaptive ? std::min<size_t>(G.capU, e.U) : 1;
         size_t capSort = std::min<size_t>(G.capSort, e.sort);
         /* ties of scores across the cut are ordered in the generic kernel only */
-        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, needs, false, e.group);
-        if (L.per_wave * (64 / e.group) > 64 * 1024) continue;
+        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, needs, false, e.group,
+                                      (size_t)capJ * std::max(1, ctx->maxcolnnz));
+        if (L.per_wave * (64 / e.group) > ctx->lds_block) continue;
         out.push_back({e.group, e.wcap, L});
         if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU)) break;
     }
     return out;
+}
+
+/* The bin table of a build: the autotuner's choice for the build's precision
+ * (binning == 2), else table 0. */
+static int table_for(const mspai_ctx* ctx, const mspai_opts* o)
+{
+    const int t = ctx->tuned_table[o->precision == 1 ? 1 : 0];
+    return o->binning == 2 && t >= 0 ? t : 0;
 }
 
 static int ensure_scratch(mspai_ctx* ctx, size_t need)
@@ -870,14 +923,18 @@ static int end_launch(mspai_ctx* ctx)
     return MSPAI_OK;
 }
 
-/* HBM tie-path buffers of the LDS bins: one region per resident group. */
-static size_t tie_bytes_needed(const mspai_ctx* ctx, const std::vector<Bin>& bins, bool mixed, int fp64_mode)
+/* HBM tie-path buffers of the LDS bins: one region per resident group (the
+ * mixed chain's bins in their mode, the FP64 chain's in FP64). */
+static size_t tie_bytes_needed(const mspai_ctx* ctx, const std::vector<Bin>& bins, const std::vector<Bin>& fbins,
+                               bool mixed, int fp64_mode)
 {
     size_t need = 0;
-    for (const Bin& b : bins)
-        need = std::max(need, (size_t)ctx->num_cu *
-                                  std::max(waves_per_cu(b, fp64_mode), mixed ? waves_per_cu(b, fp64_mode == 2 ? 3 : 1) : 0) *
-                                  (64 / b.group) * b.L.tie_per_group);
+    for (const Bin& b : fbins)
+        need = std::max(need, (size_t)ctx->num_cu * waves_per_cu(b, fp64_mode) * (64 / b.group) * b.L.tie_per_group);
+    if (mixed)
+        for (const Bin& b : bins)
+            need = std::max(need, (size_t)ctx->num_cu * waves_per_cu(b, fp64_mode == 2 ? 3 : 1) * (64 / b.group) *
+                                      b.L.tie_per_group);
     return need;
 }
 
@@ -885,22 +942,27 @@ static size_t tie_bytes_needed(const mspai_ctx* ctx, const std::vector<Bin>& bin
 constexpr int kDirectFp64 = 4096;
 
 /* Launch routing and the bin chain over the columns whose patterns are in
- * a.p_ptr/p_ind.  Queues: FP64 queue b at d_queues + b*n (b = 0..nb, nb =
- * generic), mixed queue b at d_queues + (kMaxBins + 1 + b)*n.  Routing fills
- * the first queues; a bin's overflow appends to the next queue of its chain;
- * the mixed chain ends in FP64 queue 0, which also takes the columns the
- * mixed guards decline.  Counters: FP64 [0..8) lengths, [8..16) heads,
- * [16..24) routed lengths, mixed [24..32) lengths, [32..40) heads, [40]
- * blocks of the routing kernel done. */
-static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins, const ScratchLayout& G,
-                     int gwaves, bool mixed, int fp64_mode, int n)
+ * a.p_ptr/p_ind.  `bins` are the mixed chain's bins in a mixed build (else
+ * the FP64 chain's), `fbins` the FP64 chain's.  Queues: FP64 queue b at
+ * d_queues + b*n (b = 0..nf, nf = generic), mixed queue b at d_queues +
+ * (kMaxBins + 1 + b)*n.  Routing fills the first queues; a bin's overflow
+ * appends to the next queue of its chain; the mixed chain ends in FP64
+ * queue 0, which also takes the columns the mixed guards decline.  Columns
+ * go on from the step they reached (save_state) except in the adjoint and
+ * the QR update, which restart them.  Counters: FP64 [0..8) lengths,
+ * [8..16) heads, [16..24) routed lengths, mixed [24..32) lengths, [32..40)
+ * heads, [40] blocks of the routing kernel done. */
+static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins, const std::vector<Bin>& fbins,
+                     const ScratchLayout& G, int gwaves, bool mixed, int fp64_mode, int n)
 {
     const int nb = (int)bins.size();
+    const int nf = (int)fbins.size();
+    const std::vector<Bin>& routed = mixed ? bins : fbins;  // the chain the routing kernel fills
     BinEdges edges{};
-    for (int b = 0; b < nb; ++b) {
-        edges.capJ[b] = bins[b].L.capJ;
-        edges.capSort[b] = bins[b].L.capSort;
-        edges.capI[b] = bins[b].L.capI;
+    for (size_t b = 0; b < routed.size(); ++b) {
+        edges.capJ[b] = routed[b].L.capJ;
+        edges.capSort[b] = routed[b].L.capSid;  // the I-set raw list of the start pattern
+        edges.capI[b] = routed[b].L.capI;
     }
     int* counts = ctx->d_counter;
     int* heads = ctx->d_counter + 8;
@@ -915,7 +977,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     ctx->lwork_valid = a.work != nullptr;
     if (a.work) HIPCHK(hipMemsetAsync(ctx->d_lwork, 0, kMaxLaunch * kLaunchStat * sizeof(double), ctx->stream));
     a.status_or = 0;
-    const size_t tie_need = tie_bytes_needed(ctx, bins, mixed, fp64_mode);
+    const size_t tie_need = tie_bytes_needed(ctx, bins, fbins, mixed, fp64_mode);
     if (tie_need > ctx->tie_bytes) {
         if (ctx->no_alloc) return fail(ctx, MSPAI_ERR_CAPACITY, "tie buffers: this call allocates no device memory");
         dfree(ctx->d_tie);
@@ -924,6 +986,21 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     }
     a.tie_scratch = ctx->d_tie;
     if (mixed) {
+        /* FP64 bins that hold nothing the last mixed bin cannot: the mixed
+         * chain's overflow passes them by (declined columns start at bin 0) */
+        const ScratchLayout& Lm = bins[nb - 1].L;
+        int skip = 0;
+        while (skip < nf && fbins[skip].L.capJ <= Lm.capJ && fbins[skip].L.capI <= Lm.capI &&
+               fbins[skip].L.capSort <= Lm.capSort && fbins[skip].L.capU <= Lm.capU &&
+               fbins[skip].L.capSid <= Lm.capSid)
+            ++skip;
+        /* the columns for FP64 go to the generic kernel in one launch when
+         * the last mixed build had few of them and no FP64 bin is wider than
+         * the mixed ones: the FP64 bins would mostly launch on empty queues.
+         * Both routes give the same bits. */
+        const bool direct =
+            fp64_mode == 0 && skip == nf && ctx->mixed_fp64_last >= 0 && ctx->mixed_fp64_last <= kDirectFp64;
+        const int to = direct ? 0 : skip;  // FP64 queue of the mixed chain's overflow
         launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->d_counter + 40, ctx->d_counter + 16,
                         ctx->stream);
         HIPCHK(hipGetLastError());
@@ -931,8 +1008,11 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         a.n_fallback = counts;
         for (int b = 0; b < nb; ++b) {
             const int grp = bins[b].group;
-            a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq;
-            a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts;
+            a.overflow = b + 1 < nb ? mq + (size_t)(b + 1) * n : fq + (size_t)to * n;
+            /* overflow and declined steps go on from the step they reached,
+             * in the next mixed bin or in the FP64 chain */
+            a.save_state = fp64_mode != 2;
+            a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts + to;
             int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
             const ScratchLayout& L = bins[b].L;
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
@@ -944,36 +1024,34 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
         }
         a.status_or = COL_FALLBACK;
-        /* the columns for FP64 (fallbacks, and what no mixed bin holds) go
-         * to the generic kernel in one launch when the last mixed build had
-         * few of them: the FP64 bins would mostly launch on empty queues.
-         * Both routes give the same bits. */
-        if (fp64_mode == 0 && ctx->mixed_fp64_last >= 0 && ctx->mixed_fp64_last <= kDirectFp64) {
+        a.save_state = 0;
+        if (direct) {
             a.fallback = nullptr;
             a.n_fallback = nullptr;
             a.overflow = fq;  // never written: the generic layout holds every column
             a.n_overflow = counts + 7;
             if ((rc = begin_launch(ctx, a, "generic (HBM working set)", fp64_mode, -1, nullptr))) return rc;
-            launch_fp64_generic(a, G, fp64_mode, fq, counts, heads + nb, gwaves, ctx->stream);
+            launch_fp64_generic(a, G, fp64_mode, fq, counts, heads + nf, gwaves, ctx->stream);
             HIPCHK(hipGetLastError());
             return end_launch(ctx);
         }
     } else {
-        launch_classify(a, nb, edges, fq, counts, fq + (size_t)nb * n, counts + nb, n, ctx->d_counter + 40,
+        launch_classify(a, nf, edges, fq, counts, fq + (size_t)nf * n, counts + nf, n, ctx->d_counter + 40,
                         ctx->d_counter + 16, ctx->stream);
         HIPCHK(hipGetLastError());
     }
     a.fallback = nullptr;
     a.n_fallback = nullptr;
-    for (int b = 0; b < nb; ++b) {
-        const int grp = bins[b].group;
+    for (int b = 0; b < nf; ++b) {
+        const int grp = fbins[b].group;
         a.overflow = fq + (size_t)(b + 1) * n;
         a.n_overflow = counts + b + 1;
-        int per_cu = waves_per_cu(bins[b], fp64_mode);
-        const ScratchLayout& L = bins[b].L;
+        int per_cu = waves_per_cu(fbins[b], fp64_mode);
+        const ScratchLayout& L = fbins[b].L;
         const bool first = b == 0 && !mixed;
+        a.save_state = fp64_mode == 0;  // FP64 builds resume in the next bin (not the QR update or the adjoint)
         if (first) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-        if ((rc = begin_launch(ctx, a, nullptr, fp64_mode, b, &bins[b]))) return rc;
+        if ((rc = begin_launch(ctx, a, nullptr, fp64_mode, b, &fbins[b]))) return rc;
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
                       ctx->num_cu * per_cu, ctx->stream);
         HIPCHK(hipGetLastError());
@@ -982,12 +1060,13 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
     }
     a.overflow = fq;  // never written: the generic layout holds every column
     a.n_overflow = counts + 7;
-    if (nb == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));  // no bins: the generic kernel is the first
+    a.save_state = 0;
+    if (nf == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));  // no bins: the generic kernel is the first
     if ((rc = begin_launch(ctx, a, "generic (HBM working set)", fp64_mode, -1, nullptr))) return rc;
-    launch_fp64_generic(a, G, fp64_mode, fq + (size_t)nb * n, counts + nb, heads + nb, gwaves, ctx->stream);
+    launch_fp64_generic(a, G, fp64_mode, fq + (size_t)nf * n, counts + nf, heads + nf, gwaves, ctx->stream);
     HIPCHK(hipGetLastError());
     if ((rc = end_launch(ctx))) return rc;
-    if (nb == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+    if (nf == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     return MSPAI_OK;
 }
 
@@ -1011,7 +1090,7 @@ static int read_chain_counts(mspai_ctx* ctx, int nbins, bool mixed, mspai_stats*
         int moved = 0;
         for (int b = 1; b < nbins; ++b) moved += cnt[24 + b] - cnt[16 + b];
         stats->n_overflow = moved;
-        stats->n_fallback = cnt[0] - cnt[16 + nbins];
+        stats->n_fallback = cnt[62];  // columns with COL_FALLBACK, counted by the compaction
         for (int b = 0; b < 8; ++b) {
             ctx->last_routed[b] = cnt[16 + b];
             ctx->last_queued[b] = b < nbins ? cnt[24 + b] : (b == nbins ? cnt[0] : 0);
@@ -1101,10 +1180,15 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     const bool qrup = opts->qr == 2 && ns > 0 && opts->maxnew > 0;  // opt_level 3, src/imspai.cpp:483-491
     const int needs = (qrup ? kNeedQrUpdate : 0) | (opts->precision == 1 && !qrup ? kNeedMixed : 0);
     const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive, needs);
-    std::vector<Bin> bins;
+    std::vector<Bin> bins, fbins;
     if (opts->binning) {
-        int t = (opts->binning == 2 && ctx->tuned_table >= 0) ? ctx->tuned_table : 0;
+        int t = table_for(ctx, opts);
         bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t], needs);
+        /* the FP64 chain of a mixed build: the same table in FP64 layouts
+         * (no FP32 copies), which may keep bins the mixed layouts exceed */
+        fbins = (needs & kNeedMixed) ? bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t],
+                                                   needs & ~kNeedMixed)
+                                     : bins;
         ctx->last_table = t;
     }
     int gwaves = 0;
@@ -1148,7 +1232,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         a.cache_hits = ctx->d_counter + 48;
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    if ((rc = run_chain(ctx, a, bins, G, gwaves, mixed, qrup ? 4 : 0, n))) return rc;
+    if ((rc = run_chain(ctx, a, bins, fbins, G, gwaves, mixed, qrup ? 4 : 0, n))) return rc;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
 
     /* compaction: colptr[0] = 0, colptr[1..n] = inclusive scan of the lengths */
@@ -1157,7 +1241,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
                                              ctx->stream));
     hipLaunchKernelGGL(scatter_columns_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, cap,
                        ctx->d_m_len, ctx->d_m_idx, ctx->d_m_val, ctx->d_Mptr, ctx->d_Mind, ctx->d_Mval,
-                       ctx->d_counter + 63);
+                       ctx->d_counter + 63, ctx->d_status, mixed ? ctx->d_counter + 62 : nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if ((rc = fetch_counters(ctx))) return rc;
@@ -1168,7 +1252,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     ctx->pattern_nnz = last;
     ctx->m_csr_ok = false;
     ctx->last_mixed = mixed;
-    ctx->mixed_fp64_last = mixed ? ctx->h_counter[0] : -1;
+    ctx->mixed_fp64_last = mixed ? ctx->h_counter[62] : -1;
     ctx->last_binning = opts->binning;
     ctx->last_counters = opts->counters != 0;
     if (stats) {
@@ -1183,7 +1267,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
         stats->first_bin_ms = t_first;
         stats->nnz = last;
         stats->ncols = n;
-        if ((rc = read_chain_counts(ctx, (int)bins.size(), mixed, stats))) return rc;
+        if ((rc = read_chain_counts(ctx, (int)(mixed ? bins.size() : fbins.size()), mixed, stats))) return rc;
         stats->n_cache_hits = a.cache_slots > 0 ? ctx->h_counter[48] : 0;
     }
     return MSPAI_OK;
@@ -1226,13 +1310,16 @@ extern "C" int mspai_build_noalloc(mspai_ctx* ctx, const mspai_opts* opts, int* 
     const bool qrup = opts->qr == 2 && adaptive && opts->maxnew > 0;
     const int needs = (qrup ? kNeedQrUpdate : 0) | (opts->precision == 1 && !qrup ? kNeedMixed : 0);
     const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive, needs);
-    std::vector<Bin> bins;
+    std::vector<Bin> bins, fbins;
     if (opts->binning) {
-        int t = (opts->binning == 2 && ctx->tuned_table >= 0) ? ctx->tuned_table : 0;
+        int t = table_for(ctx, opts);
         bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t], needs);
+        fbins = (needs & kNeedMixed) ? bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t],
+                                                   needs & ~kNeedMixed)
+                                     : bins;
     }
     const bool mixed = opts->precision == 1 && !bins.empty() && !qrup;
-    const size_t tie = align_up(tie_bytes_needed(ctx, bins, mixed, qrup ? 4 : 0), 256);
+    const size_t tie = align_up(tie_bytes_needed(ctx, bins, fbins, mixed, qrup ? 4 : 0), 256);
     size_t cub = 0;
     hipcub::DeviceScan::InclusiveSum(nullptr, cub, (const int*)nullptr, (int*)nullptr, (int)n, ctx->stream);
     cub = align_up(std::max<size_t>(cub, 256), 256);
@@ -1354,7 +1441,7 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
     const ScratchLayout G = generic_layout(ctx, cap, 1, false, needs);
     std::vector<Bin> bins;
     if (opts->binning) {
-        int t = (opts->binning == 2 && ctx->tuned_table >= 0) ? ctx->tuned_table : 0;
+        int t = table_for(ctx, opts);
         bins = bin_layouts(ctx, cap, 1, false, G, kTables[t], needs);
     }
     int gwaves = 0;
@@ -1372,7 +1459,7 @@ extern "C" int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const doub
     a.cap = cap;
     const bool mixed = opts->precision == 1 && !bins.empty();
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-    if ((rc = run_chain(ctx, a, bins, G, gwaves, mixed, 2, n))) return rc;
+    if ((rc = run_chain(ctx, a, bins, bins, G, gwaves, mixed, 2, n))) return rc;
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if ((rc = fetch_counters(ctx))) return rc;
@@ -1404,14 +1491,20 @@ extern "C" int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeat
     int best = 0;
     double best_ms = 1e300;
     if (report) std::memset(report, 0, sizeof(*report));
+    const int prec = o.precision == 1 ? 1 : 0;
     for (int t = 0; t < kNumTables && t < MSPAI_MAX_TUNE; ++t) {
-        ctx->tuned_table = t;
+        if (prec == 1 && kTables[t].count == 0) {
+            /* the mixed kernels run in size bins: a bin-less table is all FP64 */
+            if (report) report->candidate_ms[t] = -1.0;
+            continue;
+        }
+        ctx->tuned_table[prec] = t;
         double tmin = 1e300;
         for (int r = 0; r < repeats + 1; ++r) {  // first run warms up
             mspai_stats st;
             int rc = mspai_build(ctx, &o, &st);
             if (rc) {
-                ctx->tuned_table = -1;
+                ctx->tuned_table[prec] = -1;
                 return rc;
             }
             if (r > 0) tmin = std::min(tmin, st.solve_ms);
@@ -1422,12 +1515,26 @@ extern "C" int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeat
             best = t;
         }
     }
-    ctx->tuned_table = best;
+    ctx->tuned_table[prec] = best;
     if (report) {
         report->ncandidates = std::min(kNumTables, MSPAI_MAX_TUNE);
         report->chosen = best;
         std::snprintf(report->chosen_name, sizeof(report->chosen_name), "%s", kTables[best].name);
     }
+    return MSPAI_OK;
+}
+
+extern "C" int mspai_set_bin_table(mspai_ctx* ctx, int precision, int table, char* name, int* ntables)
+{
+    if (!ctx) return MSPAI_ERR_ARG;
+    if (ntables) *ntables = kNumTables;
+    if (table < 0) return MSPAI_OK;  // the count only
+    if (table >= kNumTables || precision < 0 || precision > 1)
+        return fail(ctx, MSPAI_ERR_ARG, "set_bin_table: table or precision out of range");
+    if (precision == 1 && kTables[table].count == 0)
+        return fail(ctx, MSPAI_ERR_UNSUPPORTED, "set_bin_table: mixed builds run in size bins");
+    ctx->tuned_table[precision] = table;
+    if (name) std::snprintf(name, 32, "%s", kTables[table].name);
     return MSPAI_OK;
 }
 

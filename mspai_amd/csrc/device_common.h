@@ -61,6 +61,7 @@ struct ScratchLayout {
     int capI;      // |I| (also the leading dimension of A(I,J))
     int capU;      // distinct candidates |U|
     int capSort;   // raw index lists before sorting (a power of two)
+    int capSid;    // flat entries of the columns of J (Get_I_Set's raw list, the A(I,J) gather)
     int capPairs;  // candidates on the std::sort tie path
     int ldq;       // leading dimension of aq32: capI (FP32 copy of A(I,J)), or
                    // capJ in register bins, where aq32 holds only the FP32 R
@@ -508,6 +509,14 @@ struct Group {
     static __device__ __forceinline__ int sum_int(int p)
     {
         return allreduce(p, [](int x, int y) { return x + y; });
+    }
+    static __device__ __forceinline__ int min_int(int p)
+    {
+        return allreduce(p, [](int x, int y) { return x < y ? x : y; });
+    }
+    static __device__ __forceinline__ int max_int(int p)
+    {
+        return allreduce(p, [](int x, int y) { return x > y ? x : y; });
     }
     /* 32-bit value of group lane `src` (a compile-time lane): readlane for
      * a whole wave, DPP row_newbcast (lane SRC of each row of 16, a VALU

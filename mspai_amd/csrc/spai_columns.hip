@@ -167,26 +167,153 @@ __device__ __forceinline__ int load_segments(const BuildArgs& a, const WaveScrat
     return carry;
 }
 
-/* Get_I_Set (src/Spai_Sub.imp:65-122): sorted distinct rows of the columns in
- * J.  Returns |I|, or kOverflow when the raw list or I exceeds the layout.
- * Leaves the column segments of J for gather_ahat. */
+/* Sorted distinct values of src[0..total) that are not in excl[0..nexcl),
+ * written to out, without a sort: every value lies in [lo, hi] and sets its
+ * bit of a bitmap bm of at least (hi - lo) / 32 + 1 words of LDS; the
+ * excluded values clear theirs; the set bits are read out in increasing
+ * order.  Returns the count, or kOverflow (before any write past out[cap))
+ * when it exceeds cap. */
 template <int G>
+__device__ __forceinline__ int bitmap_unique(const int* src, int total, int lo, int hi, const int* excl, int nexcl,
+                                             int* out, int cap, unsigned* bm)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    const int words = ((hi - lo) >> 5) + 1;
+    for (int w = lane; w < words; w += G) bm[w] = 0u;
+    __syncthreads();
+    for (int e = lane; e < total; e += G) {
+        const int d = src[e] - lo;
+        atomicOr(&bm[d >> 5], 1u << (d & 31));
+    }
+    __syncthreads();
+    for (int t = lane; t < nexcl; t += G) {
+        const int v = excl[t];
+        if (v >= lo && v <= hi) atomicAnd(&bm[(v - lo) >> 5], ~(1u << ((v - lo) & 31)));
+    }
+    __syncthreads();
+    int c = 0;
+    for (int w0 = 0; w0 < words; w0 += G) {
+        const int w = w0 + lane;
+        unsigned bits = w < words ? bm[w] : 0u;
+        const int cnt = __popc(bits);
+        const int incl = Gr::scan_incl(cnt);
+        const int chunk = Gr::bcast_last(incl);
+        if (c + chunk > cap) return kOverflow;
+        int o = c + incl - cnt;
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            out[o++] = lo + (w << 5) + k;
+        }
+        c += chunk;
+    }
+    __syncthreads();
+    return c;
+}
+
+/* LDS words for bitmap_unique: in a size bin the score region (free outside
+ * the scoring, and aliased with A(I,J) only while A(I,J) is dead), in the
+ * generic kernel the LDS hash region (its working set is in HBM, where
+ * atomics in L2 and reads through the vector L1 would not meet). */
+template <bool kHbm>
+__device__ __forceinline__ unsigned* bitmap_words(const ScratchLayout& L, const WaveScratch& s, int* avail)
+{
+    if (kHbm) {
+        *avail = s.htab ? s.hslots : 0;
+        return s.htab;
+    }
+    *avail = 2 * L.capU;
+    return reinterpret_cast<unsigned*>(s.rho);
+}
+
+/* Get_I_Set (src/Spai_Sub.imp:65-122): sorted distinct rows of the columns in
+ * J (by a bitmap over their range when it fits, else a sort).  Returns |I|,
+ * or kOverflow when the raw list or I exceeds the layout.  Leaves the column
+ * segments of J for gather_ahat.  kHbm: the working set is in HBM. */
+template <int G, bool kHbm>
 __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int n)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
-    const int total = load_segments<G>(a, s, n, L.capSort);
-    if (total > L.capSort) return kOverflow;
+    const int total = load_segments<G>(a, s, n, L.capSid);
+    if (total > L.capSid) return kOverflow;
+    int vmin = 0x7fffffff, vmax = -1;
     for (int e = lane; e < total; e += G) {
         const int t = s.sid[e];
-        s.sbuf[e] = a.A.rowind[s.segs[t] + e - s.segoff[t]];
+        const int v = a.A.rowind[s.segs[t] + e - s.segoff[t]];
+        s.sbuf[e] = v;
+        vmin = min(vmin, v);
+        vmax = max(vmax, v);
     }
     __syncthreads();
+    vmin = Gr::min_int(vmin);
+    vmax = Gr::max_int(vmax);
+    int avail = 0;
+    unsigned* const bm = bitmap_words<kHbm>(L, s, &avail);
+    if (total > 0 && ((vmax - vmin) >> 5) < avail)
+        return bitmap_unique<G>(s.sbuf, total, vmin, vmax, nullptr, 0, s.I, L.capI, bm);
     int m = group_sort_unique<G>(s.sbuf, total, s.sbuf);
     if (m > L.capI) return kOverflow;
     for (int i = lane; i < m; i += G) s.I[i] = s.sbuf[i];
     __syncthreads();
     return m;
+}
+
+/* Row -> position in I for the candidate scores (and, in the generic kernel,
+ * for the gather and the residual): open addressing (linear probing,
+ * Fibonacci hashing) over s.hslots slots of LDS -- in a size bin the
+ * right-hand-side copy b0, which is dead from the residual to the next
+ * right-hand side; in the generic kernel (working set in HBM, where a binary
+ * search of I is a chain of HBM reads) a region of LDS of its own, so that
+ * the table is never written by atomics in L2 and read back through the
+ * vector L1.  Entries pack the row above the
+ * position; at most half of the slots are used.  When I does not fit (or
+ * rows need more bits than the packing leaves) the lookups binary-search
+ * the sorted I; a probe sequence is also cut off after every slot. */
+struct RowHash {
+    unsigned* tab;  // nullptr: binary search
+    int bits, pbits;
+};
+
+template <int G>
+__device__ __forceinline__ RowHash row_hash_build(const WaveScratch& s, const ScratchLayout& L, int m, int nrows)
+{
+    const int lane = Group<G>::lane();
+    RowHash h{nullptr, 31 - __clz(s.hslots > 1 ? s.hslots : 1), 32 - __clz(L.capI > 2 ? L.capI - 1 : 1)};
+    const int nslots = 1 << h.bits;
+    if (!s.htab || 2 * m > nslots || (unsigned)nrows > (0xffffffffu >> h.pbits)) return h;
+    h.tab = s.htab;
+    for (int i = lane; i < nslots; i += G) h.tab[i] = 0xffffffffu;
+    __syncthreads();
+    for (int i = lane; i < m; i += G) {
+        const unsigned row = (unsigned)s.I[i];
+        unsigned slot = (row * 0x9e3779b1u) >> (32 - h.bits);
+        /* at least half of the slots stay empty: a free one is found */
+        for (int probe = 0; probe < nslots; ++probe) {
+            if (atomicCAS(&h.tab[slot], 0xffffffffu, (row << h.pbits) | (unsigned)i) == 0xffffffffu) break;
+            slot = (slot + 1) & (unsigned)(nslots - 1);
+        }
+    }
+    __syncthreads();
+    return h;
+}
+
+/* Position of `row` in I, or -1. */
+__device__ __forceinline__ int row_position(const RowHash& h, const int* I, int m, int row)
+{
+    if (h.tab) {
+        unsigned slot = ((unsigned)row * 0x9e3779b1u) >> (32 - h.bits);
+        const unsigned mask = (1u << h.bits) - 1u;
+        for (int probe = 0; probe <= (int)mask; ++probe) {
+            const unsigned e = h.tab[slot];
+            if (e == 0xffffffffu) return -1;
+            if ((e >> h.pbits) == (unsigned)row) return (int)(e & ((1u << h.pbits) - 1u));
+            slot = (slot + 1) & mask;
+        }
+    }
+    const int p = lower_bound_i(I, m, row);
+    return (p < m && I[p] == row) ? p : -1;
 }
 
 /* Create_Submatrix_AHat (src/Spai_Sub.imp:221-282): A(I,J) column-major with
@@ -195,7 +322,7 @@ __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& 
  * Uses the column segments build_I left for this J. */
 template <int G, bool kMixed>
 __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
-                                            int m, int n)
+                                            int m, int n, const RowHash& h = RowHash{nullptr, 0, 0})
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
@@ -210,7 +337,7 @@ __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLay
     for (int e = lane; e < total; e += G) {
         const int t = s.sid[e];
         const int src = s.segs[t] + e - s.segoff[t];
-        const int pos = lower_bound_i(s.I, m, a.A.rowind[src]);
+        const int pos = row_position(h, s.I, m, a.A.rowind[src]);  // every row of J's columns is in I
         const double v = a.A.val[src];
         s.aqr[t * ld + pos] = v;
         if (kMixed) s.aq32[t * ld + pos] = (float)v;
@@ -1229,7 +1356,8 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
  * unchanged). */
 template <int G>
 __device__ __forceinline__ double residual_norm(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
-                                                int m, int n, bool dense)
+                                                int m, int n, bool dense,
+                                                const RowHash& h = RowHash{nullptr, 0, 0})
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
@@ -1255,7 +1383,7 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
             int j = s.J[jj];
             int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
             for (int e = lane; e < cnt; e += G) {
-                int pos = lower_bound_i(s.I, m, a.A.rowind[e0 + e]);
+                int pos = row_position(h, s.I, m, a.A.rowind[e0 + e]);
                 s.r[pos] = __fma_rn(xj, a.A.val[e0 + e], s.r[pos]);
             }
             __syncthreads();
@@ -1272,67 +1400,13 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
     return sqrt(sum);
 }
 
-/* Row -> position in I for the candidate scores: open addressing (linear
- * probing, Fibonacci hashing) over s.hslots slots of LDS -- in a size bin the
- * right-hand-side copy b0, which is dead from the residual to the next
- * right-hand side; in the generic kernel (working set in HBM) a region of
- * LDS of its own, so that the table is never written by atomics in L2 and
- * read back through the vector L1.  Entries pack the row above the
- * position; at most half of the slots are used.  When I does not fit (or
- * rows need more bits than the packing leaves) the lookups binary-search
- * the sorted I; a probe sequence is also cut off after every slot. */
-struct RowHash {
-    unsigned* tab;  // nullptr: binary search
-    int bits, pbits;
-};
-
-template <int G>
-__device__ __forceinline__ RowHash row_hash_build(const WaveScratch& s, const ScratchLayout& L, int m, int nrows)
-{
-    const int lane = Group<G>::lane();
-    RowHash h{nullptr, 31 - __clz(s.hslots > 1 ? s.hslots : 1), 32 - __clz(L.capI > 2 ? L.capI - 1 : 1)};
-    const int nslots = 1 << h.bits;
-    if (!s.htab || 2 * m > nslots || (unsigned)nrows > (0xffffffffu >> h.pbits)) return h;
-    h.tab = s.htab;
-    for (int i = lane; i < nslots; i += G) h.tab[i] = 0xffffffffu;
-    __syncthreads();
-    for (int i = lane; i < m; i += G) {
-        const unsigned row = (unsigned)s.I[i];
-        unsigned slot = (row * 0x9e3779b1u) >> (32 - h.bits);
-        /* at least half of the slots stay empty: a free one is found */
-        for (int probe = 0; probe < nslots; ++probe) {
-            if (atomicCAS(&h.tab[slot], 0xffffffffu, (row << h.pbits) | (unsigned)i) == 0xffffffffu) break;
-            slot = (slot + 1) & (unsigned)(nslots - 1);
-        }
-    }
-    __syncthreads();
-    return h;
-}
-
-/* Position of `row` in I, or -1. */
-__device__ __forceinline__ int row_position(const RowHash& h, const int* I, int m, int row)
-{
-    if (h.tab) {
-        unsigned slot = ((unsigned)row * 0x9e3779b1u) >> (32 - h.bits);
-        const unsigned mask = (1u << h.bits) - 1u;
-        for (int probe = 0; probe <= (int)mask; ++probe) {
-            const unsigned e = h.tab[slot];
-            if (e == 0xffffffffu) return -1;
-            if ((e >> h.pbits) == (unsigned)row) return (int)(e & ((1u << h.pbits) - 1u));
-            slot = (slot + 1) & mask;
-        }
-    }
-    const int p = lower_bound_i(I, m, row);
-    return (p < m && I[p] == row) ? p : -1;
-}
-
 /* Augment_Sparsity (src/Spai_Sub.imp:1010-1116) with Union_Nl_Sets and
  * Compute_Rhos.  Returns the new |J|, -1 when J~ is empty, or kOverflow.
  * Without kTies a tie of scores across the cut also returns kOverflow: the
  * std::sort emulation that orders such ties lives only in the generic
  * kernel, which keeps the size bins' register budget small (ties at the cut
  * are rare: none in the five workloads, a handful in the CSparse tests). */
-template <int G, bool kMixed, bool kTies>
+template <int G, bool kMixed, bool kTies, int kB, int kR>
 __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int col,
                                        int m, int n, double rn, int* st, double* wk, int* near = nullptr)
 {
@@ -1354,8 +1428,8 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
     total = Gr::sum_int(total);
     acc3 = Gr::sum_fixed(acc3);
     if (total > L.capSort) return kOverflow;
-    /* gather the row patterns of L */
-    int off = 0;
+    /* gather the row patterns of L (and the range of their indices) */
+    int off = 0, vmin = 0x7fffffff, vmax = -1;
     for (int base = 0; base < nl; base += G) {
         int li = base + lane;
         int l = li < m ? s.I[li] : col;
@@ -1373,63 +1447,110 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
             for (int q = 0; q < 4; ++q) v[q] = e + q < len ? a.rp_ind[e0 + e + q] : 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (e + q < len) s.sbuf[off + excl + e + q] = v[q];
+                if (e + q < len) {
+                    s.sbuf[off + excl + e + q] = v[q];
+                    vmin = min(vmin, v[q]);
+                    vmax = max(vmax, v[q]);
+                }
         }
         off += Gr::bcast_last(incl);
     }
     __syncthreads();
-    int nu = group_sort_unique<G>(s.sbuf, off, s.sbuf);
-    if (nu > L.capU) return kOverflow;
-    /* J~ = U \ J */
+    vmin = Gr::min_int(vmin);
+    vmax = Gr::max_int(vmax);
+    /* J~ = U \ J in increasing order: through a bitmap over [vmin, vmax] in
+     * LDS when the range fits (the score region, free until the scores, in a
+     * size bin; the hash region in the generic kernel), else by sorting the
+     * raw list */
+    int avail = 0;
+    unsigned* const bm = bitmap_words<kTies>(L, s, &avail);
     int c = 0;
-    for (int base = 0; base < nu; base += G) {
-        int t = base + lane;
-        int u = t < nu ? s.sbuf[t] : 0;
-        bool keep = false;
-        if (t < nu) {
-            int p = lower_bound_i(s.J, n, u);
-            keep = !(p < n && s.J[p] == u);
+    if (off > 0 && ((vmax - vmin) >> 5) < avail) {
+        c = bitmap_unique<G>(s.sbuf, off, vmin, vmax, s.J, n, s.jt, L.capU, bm);
+        if (c == kOverflow) return kOverflow;
+    } else {
+        int nu = group_sort_unique<G>(s.sbuf, off, s.sbuf);
+        if (nu > L.capU) return kOverflow;
+        for (int base = 0; base < nu; base += G) {
+            int t = base + lane;
+            int u = t < nu ? s.sbuf[t] : 0;
+            bool keep = false;
+            if (t < nu) {
+                int p = lower_bound_i(s.J, n, u);
+                keep = !(p < n && s.J[p] == u);
+            }
+            unsigned long long mask = Gr::ballot(keep);
+            int q = c + __popcll(mask & Gr::lanes_below());
+            if (keep) s.jt[q] = u;
+            c += __popcll(mask);
         }
-        unsigned long long mask = Gr::ballot(keep);
-        int q = c + __popcll(mask & Gr::lanes_below());
-        if (keep) s.jt[q] = u;
-        c += __popcll(mask);
+        __syncthreads();
     }
-    __syncthreads();
     if (c == 0) {
         if (wk && lane == 0) wk[3] += acc3;
         return -1;
     }
-    /* rho_j = sqrt(||r||^2 - (r^T a_j)^2 / ||a_j||^2), numerator over I only */
+    /* rho_j = sqrt(||r||^2 - (r^T a_j)^2 / ||a_j||^2), numerator over I only.
+     * Each lane scores kB candidates at a time (t, t + G, ...): their column
+     * bounds first, then up to kR row indices of each per round, all in
+     * flight together; only the entries whose row is in I (a small share)
+     * load their value.  Every numerator still adds its column's entries in
+     * order. */
     const RowHash h = row_hash_build<G>(s, L, m, a.A.nrows);
     bool rho_near0 = false;  // mixed path: rho^2 within rounding of 0 (its sign decides NaN or not)
-    for (int t = lane; t < c; t += G) {
-        int j = s.jt[t];
-        double num = 0.0;
-        int e0 = a.A.colptr[j], e1 = a.A.colptr[j + 1];
-        /* entries in order, four loads in flight per round */
-        for (int e = e0; e < e1; e += 4) {
-            int rows[4];
-            double vals[4];
+    const double rn2 = __dmul_rn(rn, rn);
+    for (int t0 = lane; t0 < c; t0 += kB * G) {
+        int e0[kB], len[kB];
+        double ajn[kB], num[kB];
+        int lmax = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                rows[q] = e + q < e1 ? a.A.rowind[e + q] : 0;
-                vals[q] = e + q < e1 ? a.A.val[e + q] : 0.0;
+        for (int k = 0; k < kB; ++k) {
+            const int t = t0 + k * G;
+            e0[k] = 0;
+            len[k] = 0;
+            ajn[k] = 1.0;
+            num[k] = 0.0;
+            if (t < c) {
+                const int j = s.jt[t];
+                e0[k] = a.A.colptr[j];
+                len[k] = a.A.colptr[j + 1] - e0[k];
+                ajn[k] = a.ajn[j];
             }
+            lmax = max(lmax, len[k]);
+        }
+        for (int base = 0; base < lmax; base += kR) {
+            int pos[kB][kR];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (e + q < e1) {
-                    const int p = row_position(h, s.I, m, rows[q]);
-                    if (p >= 0) num = __dadd_rn(num, __dmul_rn(s.r[p], vals[q]));
-                }
+            for (int k = 0; k < kB; ++k)
+#pragma unroll
+                for (int q = 0; q < kR; ++q) pos[k][q] = base + q < len[k] ? a.A.rowind[e0[k] + base + q] : -1;
+#pragma unroll
+            for (int k = 0; k < kB; ++k)
+#pragma unroll
+                for (int q = 0; q < kR; ++q)
+                    if (pos[k][q] >= 0) pos[k][q] = row_position(h, s.I, m, pos[k][q]);
+            double v[kB][kR];
+#pragma unroll
+            for (int k = 0; k < kB; ++k)
+#pragma unroll
+                for (int q = 0; q < kR; ++q) v[k][q] = pos[k][q] >= 0 ? a.A.val[e0[k] + base + q] : 0.0;
+#pragma unroll
+            for (int k = 0; k < kB; ++k)
+#pragma unroll
+                for (int q = 0; q < kR; ++q)
+                    if (pos[k][q] >= 0) num[k] = __dadd_rn(num[k], __dmul_rn(s.r[pos[k][q]], v[k][q]));
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const int t = t0 + k * G;
+            if (t < c) {
+                const double n2 = __dmul_rn(num[k], num[k]);
+                const double rho = __dsub_rn(rn2, __ddiv_rn(n2, __dmul_rn(ajn[k], ajn[k])));
+                if (kMixed) rho_near0 |= rho <= kGuard * rn * rn;
+                s.rho[t] = sqrt(rho);
+                acc4 += 16.0 + 12.0 * len[k];
             }
         }
-        num = __dmul_rn(num, num);
-        double ajn = a.ajn[j];
-        double rho = __dsub_rn(__dmul_rn(rn, rn), __ddiv_rn(num, __dmul_rn(ajn, ajn)));
-        if (kMixed) rho_near0 |= rho <= kGuard * rn * rn;
-        s.rho[t] = sqrt(rho);
-        acc4 += 16.0 + 12.0 * (e1 - e0);
     }
     __syncthreads();
     if (kMixed && Gr::ballot(rho_near0) != 0ull) return kFallback;
@@ -1605,7 +1726,7 @@ __device__ __forceinline__ void count_solve(const BuildArgs& a, const WaveScratc
     wk[7] += dn;
 }
 
-template <int G, bool kMixed, int NMAX>
+template <int G, bool kMixed, int NMAX, bool kHbm>
 __device__ __forceinline__ int backward_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                                int col)
 {
@@ -1621,7 +1742,7 @@ __device__ __forceinline__ int backward_column(const BuildArgs& a, const Scratch
         s.g[i] = a.gM[p0 + i];
     }
     __syncthreads();
-    int m = build_I<G>(a, L, s, n);
+    int m = build_I<G, kHbm>(a, L, s, n);
     if (m == kOverflow) return kMoveOn;
     if (m < n) return kDone;  // degenerate column: no gradient
     gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n);
@@ -1739,7 +1860,7 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
     __syncthreads();
     int st = 0, solves = 0;
     /* first factorization: QRZerlegungLapack */
-    int m = build_I<G>(a, L, s, n);
+    int m = build_I<G, kTies>(a, L, s, n);
     if (m == kOverflow) return kMoveOn;
     st |= make_rhs<G>(a, s, m, col);
     gather_ahat<G, false>(a, L, s, m, n);
@@ -1809,12 +1930,12 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
         }
         __syncthreads();
         const int nold = n, mold = m;
-        int nn = augment<G, false, kTies>(a, L, s, col, m, n, rn, &st, wk);
+        int nn = augment<G, false, kTies, 1, 8>(a, L, s, col, m, n, rn, &st, wk);
         if (nn == kOverflow) return kMoveOn;
         if (nn < 0) break;
         n = nn;
         if (n > L.capJ) return kMoveOn;
-        m = build_I<G>(a, L, s, n);
+        m = build_I<G, kTies>(a, L, s, n);
         if (m == kOverflow) return kMoveOn;
         st |= make_rhs<G>(a, s, m, col);
         /* storage positions: old rows keep theirs, new rows go to the bottom in
@@ -2103,14 +2224,19 @@ __device__ __forceinline__ int ls_solve_cached(const BuildArgs& a, const Scratch
     return st_solve;
 }
 
-/* A column that outgrows the launch's layout during `step` leaves the state
- * of that step's start for the next launch: J in its output slot, |J| in
- * m_len, the status bits, nsolve = -(step + 2) as the mark, and in builds
- * with counters the work counters in their row and the near-margin counts
- * packed into res (each below 2^20; read back exactly).  Every value the
- * step computes depends only on these, so the resumed column is bitwise the
- * column built in one launch.  Without save_state (a mixed launch whose
- * overflow goes on in FP64) the mark is 0: the next launch starts afresh. */
+/* A column that outgrows the launch's layout during `step`, or whose step a
+ * guard of the mixed path declines, leaves the state of that step's start
+ * for the next launch: J in its output slot, |J| in m_len, the status bits,
+ * nsolve = -(step + 2) as the mark, and in builds with counters the work
+ * counters in their row and the near-margin counts packed into res (each
+ * below 2^20; read back exactly).  Every value the step computes depends
+ * only on these, so the resumed column is bitwise the column built in one
+ * launch.  A mixed launch leaves the FP64 state: every earlier step cleared
+ * the guards, so its decisions were FP64's (and none was within the
+ * near-margin band, which lies inside the guard band); index sets, work
+ * counters and the status bits a mixed step sets (the target column outside
+ * I) do not depend on the precision.  Without save_state the mark is 0: the
+ * next launch starts afresh. */
 template <int G>
 __device__ __forceinline__ void save_column_state(const BuildArgs& a, const WaveScratch& s, int col, int step, int n,
                                                   int st, const int (&near)[3], double wk, bool counters)
@@ -2178,6 +2304,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
     /* the state at the start of the current step, left for the next launch
      * if the column outgrows this one's layout during the step */
     int moved = -2, st0 = st, near0[3] = {near[0], near[1], near[2]};
+    bool redo = false;  // mixed: a guard declined the step, which FP64 redoes
     double wk0 = 0.0;
     for (int step = step0; step < a.ns; ++step) {
         st0 = st;
@@ -2185,16 +2312,22 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         near0[1] = near[1];
         near0[2] = near[2];
         if (wk && lane < kNumWork) wk0 = wk[lane];
-        int m = build_I<G>(a, L, s, n);
+        int m = build_I<G, kTies>(a, L, s, n);
         if (m == kOverflow) {
             moved = step;
             break;
         }
-        gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n);
+        /* the generic kernel (I in HBM) finds rows through the LDS hash */
+        const RowHash hI = kTies ? row_hash_build<G>(s, L, m, a.A.nrows) : RowHash{nullptr, 0, 0};
+        gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n, hI);
         st |= make_rhs<G>(a, s, m, col);
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {
-            if (ls_solve_mixed<G, NMAX>(a, L, s, m, n) == kFallback) return kRedoFp64;
+            if (ls_solve_mixed<G, NMAX>(a, L, s, m, n) == kFallback) {
+                redo = true;
+                moved = step;
+                break;
+            }
         } else if (NMAX > 0 && m <= G && n <= NMAX) {
             /* the register QR costs less than a cache lookup (which reads the
              * stored block and factor back from HBM): -cs serves the other
@@ -2211,22 +2344,31 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         }
         ++solves;
         count_solve<G>(a, s, m, n, wk);
-        rn = residual_norm<G>(a, L, s, m, n, dense);
-        if (kMixed && !(fabs(rn - a.epsilon) > kGuard * a.epsilon)) return kRedoFp64;
+        rn = residual_norm<G>(a, L, s, m, n, dense, hI);
+        if (kMixed && !(fabs(rn - a.epsilon) > kGuard * a.epsilon)) {
+            redo = true;
+            moved = step;
+            break;
+        }
         if (!kMixed && wk) near[2] += fabs(rn - a.epsilon) <= kNearMargin * a.epsilon;
         if (rn <= a.epsilon || step == a.ns - 1) break;
-        int nn = augment<G, kMixed, kTies>(a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
+        int nn = augment<G, kMixed, kTies, (G == 64 && NMAX == 0 && !kTies) ? 2 : 1,
+                         (G == 64 && !kTies) ? 16 : 8>(a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
         if (nn == kOverflow) {
             moved = step;  // J is unchanged: augment merges only once every check has passed
             break;
         }
-        if (nn == kFallback) return kRedoFp64;
+        if (nn == kFallback) {
+            redo = true;
+            moved = step;
+            break;
+        }
         if (nn < 0) break;
         n = nn;
     }
     if (moved > -2) {
         if (kMixed || a.save_state) save_column_state<G>(a, s, col, moved, n, st0, near0, wk0, wk != nullptr);
-        return kMoveOn;
+        return redo ? kRedoFp64 : kMoveOn;
     }
     for (int i = lane; i < n; i += G) {
         a.m_idx[o + i] = s.J[i];
@@ -2267,7 +2409,7 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
  * waves per CU. */
 template <int G, bool kLds, int kMode, int NMAX>
 __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)
-                                : ((G == 64 && NMAX == 32 && kMode == 0) ? 3
+                                : ((G == 64 && NMAX == 32) ? (kMode == 0 ? 3 : 2)
                                    : ((kLds && kMode < 2 && (G == 16 || (G == 32 && NMAX == 16))) ? 3
                                       : (!kLds ? 4 : 1)))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
@@ -2336,14 +2478,12 @@ __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)
                 if (kMode < 2)
                     outcome = solve_column<G, kMode == 1, NMAX, !kLds>(a, L, s, col);
                 else if (kMode < 4)
-                    outcome = backward_column<G, kMode == 3, NMAX>(a, L, s, col);
+                    outcome = backward_column<G, kMode == 3, NMAX, !kLds>(a, L, s, col);
                 else
                     outcome = solve_column_qrup<G, !kLds>(a, L, s, col);
             }
             __syncthreads();
             const bool leader = Group<G>::lane() == 0;
-            /* a column the mixed path declines starts afresh in FP64 */
-            if (kMode == 1 && leader && outcome == kRedoFp64) a.nsolve[col] = 0;
             wave_append(leader && outcome == kMoveOn, col, a.overflow, a.n_overflow);
             if (kMode == 1 || kMode == 3) wave_append(leader && outcome == kRedoFp64, col, a.fallback, a.n_fallback);
         }

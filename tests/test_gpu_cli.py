@@ -75,3 +75,61 @@ def test_cli_block_and_hash(oracle, tmp_path, name):
                         max_impr_steps=ns, threads=THREADS)
     assert out.read_text() == format_mtx(OB.to_scalar(B, ref))
 
+
+
+PROBE_CASES = {
+    # name: (system A, probe rows Ce (None: e^T A), Be (None: e^T), rho, target T or None, -pp)
+    "laplace3d_6_e": (lambda: gen.laplace3d(6), None, None, 100.0, None, 2),
+    "convdiff_12_random2_target": (lambda: gen.config2(12).A, "random", "random", 1.75, "random", 1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PROBE_CASES))
+def test_cli_probing_against_oracle(oracle, tmp_path, name):
+    """-Ce/-Be/-rho and a target file through the command line (Read_Matrix_File
+    and Read_Target_File, src/Read_mm_Matrix.cpp:115-420; the static solve of
+    MSPAI probing, src/Spai_Sub.imp): C = [A; rho Ce] and B = [T or I; rho Be],
+    the start pattern taken from the rows of A only (Matrix::To_Pattern,
+    src/Matrix.imp:285-356). The written M equals the oracle's M for the same C,
+    B and pattern, byte for byte."""
+    import numpy as np
+    import scipy.sparse as sp
+    make, ce, be, rho, tgt, pp = PROBE_CASES[name]
+    A = sp.csc_matrix(make())
+    n = A.shape[1]
+    rng = np.random.RandomState(7)
+    if ce is None:
+        e = np.ones(n)
+        Ce = sp.csr_matrix(np.asarray(e @ A).reshape(1, n))
+        Be = sp.csr_matrix(e.reshape(1, n))
+    else:
+        Ce = sp.random(2, n, density=0.3, random_state=rng, format="csr")
+        Be = sp.random(2, n, density=0.2, random_state=rng, format="csr")
+    Ce.eliminate_zeros()
+    Be.eliminate_zeros()
+    T = None
+    if tgt is not None:
+        T = (sp.identity(n, format="csc") + sp.random(n, n, density=0.02, random_state=rng)).tocsc()
+        T.eliminate_zeros()
+    files = {}
+    for key, M in (("a", A), ("ce", Ce), ("be", Be), ("t", T)):
+        if M is not None:
+            files[key] = str(tmp_path / f"{key}.mtx")
+            write_input_mtx(files[key], M)
+    out = tmp_path / "precond.mtx"
+    args = [CLI, files["a"], f"-{pp}"]
+    if T is not None:
+        args.append(files["t"])
+    args += ["-Ce", files["ce"], "-Be", files["be"], "-rho", repr(rho), "-ns", "0", "-ep", "0.4", "-mn", "5",
+             "-out", str(out)]
+    print(f"[cli probing {name}] {' '.join(args[2:])}", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    C = sp.vstack([A, Ce.multiply(rho)]).tocsc()
+    B = sp.vstack([T if T is not None else sp.identity(n), Be.multiply(rho)]).tocsc()
+    C.sort_indices()
+    B.sort_indices()
+    P = gen.pattern_of(A) if pp == 2 else gen.diag_pattern(n)
+    prob = gen.Problem(name, C, P, B, epsilon=0.4, maxnew=5, max_impr_steps=0)
+    ref = oracle.spai(prob, threads=THREADS)
+    assert out.read_text() == format_mtx(oracle.to_csc(ref, n))

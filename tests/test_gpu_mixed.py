@@ -69,6 +69,31 @@ def test_mixed_full_c3(oracle, gpu):
     print(f"   worst column rel err {worst:.2e}, fallback fraction {fbfrac:.4f}", flush=True)
 
 
+def test_mixed_every_bin_table(oracle, gpu):
+    """Each bin table with size bins, mixed: the bar above on a band with the
+    bandwidth of the n = 262,144 workload (columns that outgrow a mixed bin go
+    on in the next one from the step they outgrew); the bin-less table is
+    refused for mixed builds."""
+    prob = gen.config3(n=32768, bw=2000)
+    b = gpu.Builder(0)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+        for t in range(b.set_bin_table(-1)):
+            try:
+                name = b.set_bin_table(t, precision=1)
+            except RuntimeError as e:
+                assert "size bins" in str(e)
+                continue
+            st = b.build(binning=2, precision=1, **opts)
+            print(f"[mixed table {t} {name}] {st['solve_ms']:.3f} ms", flush=True)
+            worst, fbfrac = check_mixed(oracle, prob, b.download(), b.column_info())
+            print(f"   worst column rel err {worst:.2e}, fallback fraction {fbfrac:.4f}", flush=True)
+    finally:
+        b.close()
+
+
 def test_mixed_repeat_routes_fp64_columns_alike(oracle, gpu):
     """A mixed build whose FP64 remainder (guard fallbacks) is small sends it
     to the generic kernel in the next build instead of through the FP64 bins:

@@ -283,6 +283,31 @@ def test_autotuned_bins_bitwise(oracle, gpu):
     assert_bitwise(oracle, prob, ref, M, info)
 
 
+def test_every_bin_table_bitwise(oracle, gpu):
+    """Each bin table (and the bin-less one) gives the oracle's bits on a band
+    with the bandwidth of the n = 262,144 workload, whose widest columns
+    (|I| to ~250, raw candidate lists to ~4,000) outgrow the narrow bins:
+    columns carry their state from bin to bin (resume at the step they
+    outgrew), and the wide bins hold more than 64 KB of LDS."""
+    prob = gen.config3(n=32768, bw=2000)
+    ref = oracle.spai(prob, threads=THREADS)
+    b = gpu.Builder(0)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+        for t in range(b.set_bin_table(-1)):
+            name = b.set_bin_table(t, precision=0)
+            st = b.build(binning=2, counters=1, **opts)
+            ls = b.launch_stats()
+            print(f"[table {t} {name}] {st['solve_ms']:.3f} ms; "
+                  + "; ".join(f"{x['name']}: {x['columns']:.0f}" for x in ls), flush=True)
+            assert sum(x["columns"] for x in ls) == prob.n
+            assert_bitwise(oracle, prob, ref, b.download(), b.column_info())
+    finally:
+        b.close()
+
+
 def test_build_stats_timings(gpu):
     """The build's event timings nest: the first size bin's launch lies inside
     the column-kernel window, which lies inside the whole build (FP64 and

@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--no-legs", action="store_true", help="only the main workload (profiling runs)")
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
                     help="timed paths; with both, value is the mixed path")
+    ap.add_argument("--profile", type=int, default=-1, metavar="TABLE",
+                    help="profiling runs: only `steps` builds (after `warmup`) of one precision (--precision "
+                         "fp64 or mixed) on bin table TABLE, no autotuning, legs or checks")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -358,6 +361,19 @@ def main():
 
     prob, size = _problem(args.workload, args.nx)
     n = prob.n
+    if args.profile >= 0:
+        # the dispatches of a rocprofv3 run are then exactly these builds
+        prec = 1 if args.precision == "mixed" else 0
+        b = _builder(mspai_amd, prob)
+        name = b.set_bin_table(args.profile, precision=prec)
+        opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                    use_mean=prob.use_mean, precision=prec, binning=2)
+        r = _timed(b, opts, args.steps, args.warmup)
+        b.close()
+        print(json.dumps({"profile": args.workload, "n": n, "precision": args.precision, "table": name,
+                          "builds": args.steps + args.warmup, "ms_per_build": r["ms_per_step"],
+                          "launches": r["launches"]}))
+        return
     print(f"[bench] {args.workload} n={n} nnz(A)={prob.A.nnz}", file=sys.stderr, flush=True)
     b = _builder(mspai_amd, prob)
     base = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,

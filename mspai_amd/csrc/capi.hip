@@ -666,7 +666,7 @@ static size_t next_pow2(size_t x)
 }
 
 /* Buffers a launch needs besides the FP64 build's (bit flags). */
-enum LayoutNeeds { kNeedMixed = 1, kNeedBackward = 2, kNeedQrUpdate = 4 };
+enum LayoutNeeds { kNeedMixed = 1, kNeedBackward = 2, kNeedQrUpdate = 4, kNeedA0 = 8 };
 
 /* Byte offsets of one group's working set holding the given sizes.  The
  * std::sort tie-path buffers (pairs, stack) sit in the main region when
@@ -728,6 +728,10 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.ldq = regbin ? capJ : (int)capI;
     if (needs & kNeedMixed) L.aq32 = take((size_t)L.ldq * capJ * 4);
     if (needs & kNeedBackward) L.rq = take((size_t)capJ * capJ * 8);
+    if (needs & kNeedA0) {
+        L.a0 = take(capI * capJ * 8);
+        L.keep_a0 = 1;
+    }
     /* the FP32 right-hand side of the LDS QR and the vectors of its
      * refinement, and the adjoint's vectors (the register QR and its
      * refinement keep theirs in registers) */
@@ -777,7 +781,10 @@ static ScratchLayout generic_layout(const mspai_ctx* ctx, int cap, int maxnew, b
     size_t capI = std::max<size_t>(std::min<size_t>(ctx->nrows, rawI), 1);
     size_t rawU = adaptive ? (capI + 1) * std::max(1, ctx->maxrownnz) : 1;
     size_t capU = adaptive ? std::min<size_t>(ctx->ncols, rawU) : 1;
-    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU, needs, true, 0, rawI);
+    /* the FP64 build's generic kernel keeps a copy of A(I,J) for the dense
+     * residual (its QR runs in HBM and overwrites the working copy) */
+    const int a0 = (needs & (kNeedQrUpdate | kNeedBackward)) ? 0 : kNeedA0;
+    return make_layout(cap, capI, capU, std::max(rawI, rawU), maxnew, capU, needs | a0, true, 0, rawI);
 }
 
 /* One size bin: a lane-group width, a resident-wave cap and the LDS layout

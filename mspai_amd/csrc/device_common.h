@@ -55,8 +55,9 @@ struct DevCsc {
 struct ScratchLayout {
     size_t J, I, sbuf, aqr, b, b0, r, tau, jt, rho, pairs, sel, stack, bq32, g, y, z;
     size_t srow, spos, iold, sposold, jold, colid, reflrows, bperm, yq;
-    size_t segs, segoff, sid, wsum, aq32, rq;
+    size_t segs, segoff, sid, wsum, aq32, rq, a0;
     size_t per_wave;
+    int keep_a0;   // a0: an untouched copy of A(I,J) for the residual (the generic kernel's HBM QR overwrites aqr)
     int capJ;      // |J|
     int capI;      // |I| (also the leading dimension of A(I,J))
     int capU;      // distinct candidates |U|

@@ -66,6 +66,7 @@ struct WaveScratch {
     int* sid;      // flat entry -> position t in J
     double* wsum;  // the column's work counters
     double* rq;    // adjoint: R of the register QR (capJ x capJ)
+    double* a0;    // untouched copy of A(I,J), or nullptr
     unsigned* htab;  // row -> position hash of the candidate scores: always in LDS
     int hslots;      // its slots (a power of two)
 };
@@ -105,6 +106,7 @@ __device__ __forceinline__ WaveScratch wave_scratch(char* ws, char* tie, const S
     s.sid = (int*)(ws + L.sid);
     s.wsum = (double*)(ws + L.wsum);
     s.rq = (double*)(ws + L.rq);
+    s.a0 = L.keep_a0 ? (double*)(ws + L.a0) : nullptr;
     s.htab = nullptr;  // set by the kernel (LDS)
     s.hslots = 0;
     return s;
@@ -262,12 +264,11 @@ __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& 
 
 /* Row -> position in I for the candidate scores (and, in the generic kernel,
  * for the gather and the residual): open addressing (linear probing,
- * Fibonacci hashing) over s.hslots slots of LDS -- in a size bin the
- * right-hand-side copy b0, which is dead from the residual to the next
- * right-hand side; in the generic kernel (working set in HBM, where a binary
- * search of I is a chain of HBM reads) a region of LDS of its own, so that
- * the table is never written by atomics in L2 and read back through the
- * vector L1.  Entries pack the row above the
+ * Fibonacci hashing) over s.hslots slots of LDS -- in a size bin the raw
+ * candidate list, which is dead from the union to the next I-set; in the
+ * generic kernel (working set in HBM, where a binary search of I is a chain
+ * of HBM reads) a region of LDS of its own, so that the table is never
+ * written by atomics in L2 and read back through the vector L1.  Entries pack the row above the
  * position; at most half of the slots are used.  When I does not fit (or
  * rows need more bits than the packing leaves) the lookups binary-search
  * the sorted I; a probe sequence is also cut off after every slot. */
@@ -320,16 +321,18 @@ __device__ __forceinline__ int row_position(const RowHash& h, const int* I, int 
  * leading dimension capI, straight into the QR working array (and an FP32
  * copy for the mixed path's LDS QR; the register QR converts as it loads).
  * Uses the column segments build_I left for this J. */
-template <int G, bool kMixed>
+template <int G, bool kMixed, bool kCopy = false>
 __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int m, int n, const RowHash& h = RowHash{nullptr, 0, 0})
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
     const int ld = L.capI;
+    double* const a0 = kCopy ? s.a0 : nullptr;  // the generic kernel's untouched copy
     for (int jj = 0; jj < n; ++jj)
         for (int i = lane; i < m; i += G) {
             s.aqr[jj * ld + i] = 0.0;
+            if (a0) a0[jj * ld + i] = 0.0;
             if (kMixed) s.aq32[jj * ld + i] = 0.f;
         }
     __syncthreads();
@@ -340,6 +343,7 @@ __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLay
         const int pos = row_position(h, s.I, m, a.A.rowind[src]);  // every row of J's columns is in I
         const double v = a.A.val[src];
         s.aqr[t * ld + pos] = v;
+        if (a0) a0[t * ld + pos] = v;
         if (kMixed) s.aq32[t * ld + pos] = (float)v;
     }
     __syncthreads();
@@ -1357,19 +1361,22 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
 template <int G>
 __device__ __forceinline__ double residual_norm(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                                 int m, int n, bool dense,
-                                                const RowHash& h = RowHash{nullptr, 0, 0})
+                                                const RowHash& h = RowHash{nullptr, 0, 0},
+                                                const double* Ad = nullptr)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
     if (dense) {
-        /* A(I,J) is still intact in LDS (the register QR leaves it): dgemv's
-         * own column order, r_i = fma(x_j, A_ij, r_i) for each x_j != 0 */
+        /* A(I,J) is intact in Ad (default aqr: the register QR leaves it; the
+         * generic kernel keeps a copy): dgemv's own column order,
+         * r_i = fma(x_j, A_ij, r_i) for each x_j != 0 */
         const int ld = L.capI;
+        const double* const A = Ad ? Ad : s.aqr;
         for (int i = lane; i < m; i += G) {
             double r = -s.b0[i];
             for (int jj = 0; jj < n; ++jj) {
                 const double xj = s.b[jj];
-                if (xj != 0.0) r = __fma_rn(xj, s.aqr[jj * ld + i], r);
+                if (xj != 0.0) r = __fma_rn(xj, A[jj * ld + i], r);
             }
             s.r[i] = r;
         }
@@ -1406,7 +1413,7 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
  * std::sort emulation that orders such ties lives only in the generic
  * kernel, which keeps the size bins' register budget small (ties at the cut
  * are rare: none in the five workloads, a handful in the CSparse tests). */
-template <int G, bool kMixed, bool kTies, int kB, int kR>
+template <int G, bool kMixed, bool kTies, int kB, int kR, int kA>
 __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int col,
                                        int m, int n, double rn, int* st, double* wk, int* near = nullptr)
 {
@@ -1491,14 +1498,33 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         return -1;
     }
     /* rho_j = sqrt(||r||^2 - (r^T a_j)^2 / ||a_j||^2), numerator over I only.
-     * Each lane scores kB candidates at a time (t, t + G, ...): their column
-     * bounds first, then up to kR row indices of each per round, all in
-     * flight together; only the entries whose row is in I (a small share)
-     * load their value.  Every numerator still adds its column's entries in
-     * order. */
+     * First the column bounds of every candidate of the lane, kA columns'
+     * loads in flight at a time, parked in the candidate's score slot (the
+     * lane that parks a slot reads it back and then overwrites it with the
+     * score).  Then kB candidates at a time (t, t + G, ...): up to kR row
+     * indices of each per round, all in flight together; only the entries
+     * whose row is in I (a small share) load their value.  Every numerator
+     * still adds its column's entries in order. */
     const RowHash h = row_hash_build<G>(s, L, m, a.A.nrows);
     bool rho_near0 = false;  // mixed path: rho^2 within rounding of 0 (its sign decides NaN or not)
     const double rn2 = __dmul_rn(rn, rn);
+    int2* const bounds = reinterpret_cast<int2*>(s.rho);
+    for (int t0 = lane; t0 < c; t0 += kA * G) {
+        int lo[kA], hi[kA];
+#pragma unroll
+        for (int k = 0; k < kA; ++k) {
+            const int t = t0 + k * G;
+            lo[k] = hi[k] = 0;
+            if (t < c) {
+                const int j = s.jt[t];
+                lo[k] = a.A.colptr[j];
+                hi[k] = a.A.colptr[j + 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kA; ++k)
+            if (t0 + k * G < c) bounds[t0 + k * G] = make_int2(lo[k], hi[k] - lo[k]);
+    }
     for (int t0 = lane; t0 < c; t0 += kB * G) {
         int e0[kB], len[kB];
         double ajn[kB], num[kB];
@@ -1511,10 +1537,10 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
             ajn[k] = 1.0;
             num[k] = 0.0;
             if (t < c) {
-                const int j = s.jt[t];
-                e0[k] = a.A.colptr[j];
-                len[k] = a.A.colptr[j + 1] - e0[k];
-                ajn[k] = a.ajn[j];
+                const int2 bd = bounds[t];
+                e0[k] = bd.x;
+                len[k] = bd.y;
+                ajn[k] = a.ajn[s.jt[t]];
             }
             lmax = max(lmax, len[k]);
         }
@@ -1930,7 +1956,8 @@ __device__ __forceinline__ int solve_column_qrup(const BuildArgs& a, const Scrat
         }
         __syncthreads();
         const int nold = n, mold = m;
-        int nn = augment<G, false, kTies, 1, 8>(a, L, s, col, m, n, rn, &st, wk);
+        int nn = augment<G, false, kTies, (G == 64 && !kTies) ? 2 : 1, (G == 64 && !kTies) ? 16 : 8,
+                         (G == 64 && !kTies) ? 8 : 4>(a, L, s, col, m, n, rn, &st, wk);
         if (nn == kOverflow) return kMoveOn;
         if (nn < 0) break;
         n = nn;
@@ -2319,7 +2346,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         }
         /* the generic kernel (I in HBM) finds rows through the LDS hash */
         const RowHash hI = kTies ? row_hash_build<G>(s, L, m, a.A.nrows) : RowHash{nullptr, 0, 0};
-        gather_ahat<G, kMixed && NMAX == 0>(a, L, s, m, n, hI);
+        gather_ahat<G, kMixed && NMAX == 0, kTies>(a, L, s, m, n, hI);
         st |= make_rhs<G>(a, s, m, col);
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {
@@ -2344,7 +2371,10 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         }
         ++solves;
         count_solve<G>(a, s, m, n, wk);
-        rn = residual_norm<G>(a, L, s, m, n, dense, hI);
+        /* the generic kernel's copy of A(I,J) gives the dense residual after
+         * any QR (the same bits as the sparse one) */
+        const double* Ad = (kTies && !dense) ? s.a0 : nullptr;
+        rn = residual_norm<G>(a, L, s, m, n, dense || Ad, hI, Ad);
         if (kMixed && !(fabs(rn - a.epsilon) > kGuard * a.epsilon)) {
             redo = true;
             moved = step;
@@ -2352,8 +2382,11 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         }
         if (!kMixed && wk) near[2] += fabs(rn - a.epsilon) <= kNearMargin * a.epsilon;
         if (rn <= a.epsilon || step == a.ns - 1) break;
-        int nn = augment<G, kMixed, kTies, (G == 64 && NMAX == 0 && !kTies) ? 2 : 1,
-                         (G == 64 && !kTies) ? 16 : 8>(a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
+        /* candidates in flight per lane: more where registers are plentiful
+         * (64-lane LDS kernels without the register QR) */
+        constexpr bool kWide = G == 64 && NMAX == 0 && !kTies;
+        int nn = augment<G, kMixed, kTies, kWide ? 2 : 1, (G == 64 && !kTies) ? 16 : 8, kWide ? 8 : 4>(
+            a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
         if (nn == kOverflow) {
             moved = step;  // J is unchanged: augment merges only once every check has passed
             break;
@@ -2429,10 +2462,11 @@ __global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)
     char* tie = L.tie_in_main ? base : a.tie_scratch + ((size_t)blockIdx.x * NG + g) * L.tie_per_group;
     WaveScratch s = wave_scratch(base, tie, L);
     if (kLds) {
-        /* the hash of the scores lives in b0: pow2 <= 2 capB slots of 4 bytes in capB doubles */
-        const int capB = L.capI > L.capJ ? L.capI : L.capJ;
-        s.htab = (unsigned*)s.b0;
-        s.hslots = 1 << (31 - __clz(2 * capB));
+        /* the hash of the scores lives in the raw-list buffer, dead from the
+         * candidate union to the next I-set: capSort slots (a power of two,
+         * at least 2 capI in every bin, so that I always fits at half load) */
+        s.htab = (unsigned*)s.sbuf;
+        s.hslots = L.capSort;
     } else {
         s.htab = (unsigned*)smem;
         s.hslots = kGenericHashSlots;

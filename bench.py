@@ -44,17 +44,17 @@ WORKLOADS = {
 
 
 def _traffic(name):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/pmc_<workload>.json),
-    with the kernel it was taken on, or (None, None)."""
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/pmc_<workload>.json) and the launch (and kernel) they
+    were measured on, or (None, None, None)."""
     path = os.path.join(ROOT, "profiles", f"pmc_{name}.json")
     if not os.path.exists(path):
-        return None, None
+        return None, None, None
     try:
         d = json.load(open(path))
-        return float(d["hbm_bytes_per_launch"]), d.get("kernel")
+        return float(d["hbm_bytes_per_launch"]), d.get("launch"), d.get("kernel")
     except Exception:
-        return None, None
+        return None, None, None
 
 
 def _fro_residual(A, M, B=None):
@@ -427,7 +427,9 @@ def main():
     M = main_c["M"]
     M64 = c64["M"]
     dom = _dominant(main_t, main_c)
-    traffic, traffic_kernel = _traffic(f"{args.workload}_{size}")
+    traffic, traffic_launch, traffic_kernel = _traffic(f"{args.workload}_{size}")
+    if traffic_launch != dom["kernel"]:
+        traffic = None  # measured on another launch (another bin table won the autotuning)
     fro_gpu = _fro_residual(prob.A, M, prob.B)
     mixed_check = None
     if rmx is not None:
@@ -475,7 +477,8 @@ def main():
                                        "margin": "at most 1e-12 relative"},
         "roofline": {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_kernel": traffic_kernel, **{k: v for k, v in dom.items() if k != "achieved"}},
+                     "traffic_launch": traffic_launch, "traffic_kernel": traffic_kernel,
+                     **{k: v for k, v in dom.items() if k != "achieved"}},
         "launches": _launch_table(main_t, main_c),
         "launches_fp64": _launch_table(r64, c64),
         "bins": main_t["bins"],

@@ -847,6 +847,22 @@ static const BinTable kTables[] = {
     {"g16-g64x16-g64-g64x384", 4,
      {{16, 8, 16, 64, 128, 0}, {64, 16, 64, 512, 1024, 0}, {64, 40, 128, 1024, 2048, 0},
       {64, 40, 384, 4096, 8192, 0}}},
+    /* the same with the bulk bin at 8 resident waves per CU: its FP64
+     * kernel then keeps every register (no spills to scratch) */
+    {"g64x16/occ8-g64-g64x384", 3,
+     {{64, 16, 64, 512, 1024, 8}, {64, 40, 128, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    {"g16-g64x16/occ8-g64-g64x384", 4,
+     {{16, 8, 16, 64, 128, 0}, {64, 16, 64, 512, 1024, 8}, {64, 40, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
+    /* a narrower second bin (|J| <= 24 or 20, |I| <= 96 or 112: about 26 KB,
+     * 6 resident waves per CU instead of 3) ahead of the wide ones */
+    {"g64x16-g64x24i96-g64i160-g64x384", 4,
+     {{64, 16, 64, 512, 1024, 0}, {64, 24, 96, 1024, 2048, 0}, {64, 40, 160, 2048, 4096, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
+    {"g64x16-g64x24i96-g64x384", 3,
+     {{64, 16, 64, 512, 1024, 0}, {64, 24, 96, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    {"g64x16-g64x20i112-g64x384", 3,
+     {{64, 16, 64, 512, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 
@@ -899,7 +915,7 @@ static int waves_per_cu(const Bin& b, int mode)
 {
     size_t lds = b.L.per_wave * (64 / b.group);
     int w = (int)std::min<size_t>(32, std::max<size_t>(1, (160 * 1024) / lds));
-    const int occ = binned_blocks_per_cu(b.L, b.group, mode);
+    const int occ = binned_blocks_per_cu(b.L, b.group, mode, b.wcap);
     if (occ > 0) w = std::min(w, occ);
     if (b.wcap > 0) w = std::min(w, b.wcap);
     return w;
@@ -1025,7 +1041,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
             if ((rc = begin_launch(ctx, a, nullptr, fp64_mode == 2 ? 3 : 1, b, &bins[b]))) return rc;
             launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
-                          ctx->num_cu * per_cu, ctx->stream);
+                          ctx->num_cu * per_cu, ctx->stream, bins[b].wcap);
             HIPCHK(hipGetLastError());
             if ((rc = end_launch(ctx))) return rc;
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
@@ -1060,7 +1076,7 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
         if (first) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
         if ((rc = begin_launch(ctx, a, nullptr, fp64_mode, b, &fbins[b]))) return rc;
         launch_binned(a, L, grp, fp64_mode, fq + (size_t)b * n, counts + b, heads + b,
-                      ctx->num_cu * per_cu, ctx->stream);
+                      ctx->num_cu * per_cu, ctx->stream, fbins[b].wcap);
         HIPCHK(hipGetLastError());
         if ((rc = end_launch(ctx))) return rc;
         if (first) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));

@@ -2440,11 +2440,12 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
  * despite the spill; the mixed 8-wide kernel is slower at 5), and 3 for the
  * 16-lane and the 16-wide 32-lane builds, whose bins LDS holds at 10-12
  * waves per CU. */
-template <int G, bool kLds, int kMode, int NMAX>
-__global__ __launch_bounds__(64, (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)
-                                : ((G == 64 && NMAX == 32) ? (kMode == 0 ? 3 : 2)
-                                   : ((kLds && kMode < 2 && (G == 16 || (G == 32 && NMAX == 16))) ? 3
-                                      : (!kLds ? 4 : 1)))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
+template <int G, bool kLds, int kMode, int NMAX, int kWaves = 0>
+__global__ __launch_bounds__(64, kWaves > 0 ? kWaves
+                                 : (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)
+                                 : ((G == 64 && NMAX == 32) ? (kMode == 0 ? 3 : 2)
+                                    : ((kLds && kMode < 2 && (G == 16 || (G == 32 && NMAX == 16))) ? 3
+                                       : (!kLds ? 4 : 1)))) void spai_column_kernel(BuildArgs a, ScratchLayout L, const int* queue,
                                                          const int* nqueue, int* head)
 {
     extern __shared__ __align__(16) char smem[];
@@ -2635,7 +2636,7 @@ using ColumnKernel = void (*)(BuildArgs, ScratchLayout, const int*, const int*, 
  * build) when a group holds every row of the bin (|I| <= G) and the bin's
  * |J| fits the unrolled width; 32-lane bins whose |J| is at most 8 take the
  * 8-wide unrolling (fewer registers, more resident waves). */
-static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
+static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode, int wcap)
 {
     const bool regs = mode != 4 && register_bin(group, L.capI, L.capJ);
 #define MSPAI_PICK(MODE)                                                                            \
@@ -2648,6 +2649,10 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
         if (group == 32)
             return regs ? (L.capJ <= 8 ? &spai_column_kernel<32, true, 0, 8> : &spai_column_kernel<32, true, 0, 16>)
                         : &spai_column_kernel<32, true, 0, 0>;
+        /* a bin capped at 8 resident waves per CU takes the FP64 register
+         * kernel compiled for 2 waves per SIMD (256 VGPRs, no spills)
+         * instead of 3 (168 VGPRs, some spilled) */
+        if (regs && wcap > 0 && wcap <= 8) return &spai_column_kernel<64, true, 0, 32, 2>;
         return regs ? &spai_column_kernel<64, true, 0, 32> : &spai_column_kernel<64, true, 0, 0>;
     case 1:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 1, 8> : &spai_column_kernel<16, true, 1, 0>;
@@ -2672,11 +2677,11 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode)
 #undef MSPAI_PICK
 }
 
-int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode)
+int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode, int wcap)
 {
     static std::mutex mu;
     static std::map<std::tuple<const void*, size_t>, int> memo;
-    const ColumnKernel k = binned_kernel(L, group, mode);
+    const ColumnKernel k = binned_kernel(L, group, mode, wcap);
     const size_t lds = L.per_wave * (kWave / group);
     std::lock_guard<std::mutex> lock(mu);
     auto key = std::make_tuple((const void*)k, lds);
@@ -2689,10 +2694,10 @@ int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode)
 }
 
 void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mode, const int* queue,
-                   const int* nqueue, int* head, int nwaves, hipStream_t stream)
+                   const int* nqueue, int* head, int nwaves, hipStream_t stream, int wcap)
 {
     const size_t lds = L.per_wave * (kWave / group);
-    hipLaunchKernelGGL(binned_kernel(L, group, mode), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue, nqueue,
+    hipLaunchKernelGGL(binned_kernel(L, group, mode, wcap), dim3(nwaves), dim3(kWave), lds, stream, a, L, queue, nqueue,
                        head);
 }
 

@@ -17,11 +17,11 @@ namespace mspai {
  * a.fallback; 2/3: the adjoint in FP64 / mixed; 4: FP64 build with the
  * dense QR update (-qr 2). */
 void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mode, const int* queue,
-                   const int* nqueue, int* head, int nwaves, hipStream_t stream);
+                   const int* nqueue, int* head, int nwaves, hipStream_t stream, int wcap = 0);
 
 /* Workgroups (one wave each) of that kernel that fit on one CU at once, by
  * its registers and LDS (0 if the runtime cannot tell). */
-int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode);
+int binned_blocks_per_cu(const ScratchLayout& L, int group, int mode, int wcap = 0);
 
 /* Route columns to bin queues: queue b < nbins at queues + b*n with its
  * length in counts[b]; the rest to gqueue (length gcount).  routed[0..nbins]

@@ -797,7 +797,8 @@ struct Bin {
 
 /* A size-bin table: per bin the lane-group width, the |J|, |I|, distinct
  * candidate and raw-list capacities, and a cap on resident waves per CU
- * (0 = as many as LDS allows, at most 32). */
+ * (0 = as many as LDS allows, at most 32); `mixed` > 0 limits a mixed
+ * build's FP32 chain to the first `mixed` bins (the rest run in FP64). */
 struct BinEdge {
     int group, J, I, U, sort, wcap;
 };
@@ -805,6 +806,7 @@ struct BinTable {
     const char* name;
     int count;
     BinEdge e[4];
+    int mixed;
 };
 
 /* Candidate tables of the autotuner; table 0 is the default. */
@@ -863,6 +865,13 @@ static const BinTable kTables[] = {
      {{64, 16, 64, 512, 1024, 0}, {64, 24, 96, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
     {"g64x16-g64x20i112-g64x384", 3,
      {{64, 16, 64, 512, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    /* the same two with the FP32 QR in the bulk bin only: the wider bins'
+     * FP64 kernels run faster than their mixed ones (identical tables in
+     * FP64 builds, which the autotuner skips) */
+    {"g64x16-g64x24i96-g64x384/m1", 3,
+     {{64, 16, 64, 512, 1024, 0}, {64, 24, 96, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
+    {"g64x16-g64x20i112-g64x384/m1", 3,
+     {{64, 16, 64, 512, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 
@@ -874,6 +883,7 @@ static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, b
 {
     std::vector<Bin> out;
     for (int t = 0; t < table.count; ++t) {
+        if ((needs & kNeedMixed) && table.mixed > 0 && (int)out.size() >= table.mixed) break;
         const BinEdge& e = table.e[t];
         int capJ = std::min(cap, e.J);
         size_t capI = std::min<size_t>(G.capI, e.I);
@@ -1516,8 +1526,9 @@ extern "C" int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeat
     if (report) std::memset(report, 0, sizeof(*report));
     const int prec = o.precision == 1 ? 1 : 0;
     for (int t = 0; t < kNumTables && t < MSPAI_MAX_TUNE; ++t) {
-        if (prec == 1 && kTables[t].count == 0) {
-            /* the mixed kernels run in size bins: a bin-less table is all FP64 */
+        if ((prec == 1 && kTables[t].count == 0) || (prec == 0 && kTables[t].mixed > 0)) {
+            /* the mixed kernels run in size bins: a bin-less table is all
+             * FP64; a limit on the mixed chain changes nothing in FP64 */
             if (report) report->candidate_ms[t] = -1.0;
             continue;
         }

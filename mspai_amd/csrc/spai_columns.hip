@@ -964,7 +964,7 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
         if constexpr (G == 32) {
             if (n <= 8) return refine_regs<G, 8>(L, s, m, n);
         }
-        if constexpr (G == 64 && NMAX == 32) return refine_regs<G, 32>(L, s, m, n);
+        if constexpr (G == 64 && (NMAX == 32 || NMAX == 16)) return refine_regs<G, NMAX>(L, s, m, n);
         /* the same steps with one row (r_i) or one unknown (x_i, g_i) per lane:
          * no barrier except after publishing x for the next residual */
         const bool row = lane < m;
@@ -1208,10 +1208,10 @@ __device__ __forceinline__ double rowgroup_sum(const double (&p)[4])
     return v;
 }
 
-template <bool kROut>
+template <bool kROut, int NS = 8>
 __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
-    constexpr int NS = 8;  // column slots; slot NS holds b
+    /* NS column slots per lane (|J| <= 4 NS); slot NS holds b */
     const int lane = lane_id();
     const int c = lane >> 4, q = lane & 15;
     const int ld = L.capI;
@@ -2359,8 +2359,8 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
             /* the register QR costs less than a cache lookup (which reads the
              * stored block and factor back from HBM): -cs serves the other
              * solves; both give the same bits */
-            if constexpr (G == kWave && NMAX == 32)
-                st |= ls_solve_rows64<false>(L, s, m, n);
+            if constexpr (G == kWave && (NMAX == 32 || NMAX == 16))
+                st |= ls_solve_rows64<false, NMAX / 4>(L, s, m, n);
             else
                 st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n);
             dense = true;
@@ -2653,12 +2653,19 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode, i
          * kernel compiled for 2 waves per SIMD (256 VGPRs, no spills)
          * instead of 3 (168 VGPRs, some spilled) */
         if (regs && wcap > 0 && wcap <= 8) return &spai_column_kernel<64, true, 0, 32, 2>;
+        /* |J| <= 16: the row-group QR with 4 column slots (fewer registers,
+         * less spilled at 3 waves per SIMD) */
+        if (regs && L.capJ <= 16) return &spai_column_kernel<64, true, 0, 16, 3>;
         return regs ? &spai_column_kernel<64, true, 0, 32> : &spai_column_kernel<64, true, 0, 0>;
     case 1:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 1, 8> : &spai_column_kernel<16, true, 1, 0>;
         if (group == 32)
             return regs ? (L.capJ <= 8 ? &spai_column_kernel<32, true, 1, 8> : &spai_column_kernel<32, true, 1, 16>)
                         : &spai_column_kernel<32, true, 1, 0>;
+        /* |J| <= 16: the register QR and refinement unrolled to 16 columns,
+         * compiled for 3 waves per SIMD (the 32-wide kernel needs 253 VGPRs,
+         * 2 waves per SIMD, where LDS would hold 11 per CU) */
+        if (regs && L.capJ <= 16) return &spai_column_kernel<64, true, 1, 16, 3>;
         return regs ? &spai_column_kernel<64, true, 1, 32> : &spai_column_kernel<64, true, 1, 0>;
     case 2:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 2, 8> : &spai_column_kernel<16, true, 2, 0>;

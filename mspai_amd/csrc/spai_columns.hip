@@ -429,23 +429,34 @@ __device__ __forceinline__ int ls_solve(const ScratchLayout& L, const WaveScratc
                     live[t] = c < nc;
                     cp[t] = c < nc - 1 ? s.aqr + (i + 1 + c) * ld : s.b;
                 }
+                /* a row's five loads are issued together, unconditionally
+                 * (a dead slot reads b, in bounds, and adds or stores
+                 * nothing): behind per-slot branches each load waited for
+                 * the one before it */
                 double p[4] = {0.0, 0.0, 0.0, 0.0};
                 for (int r = lane; r < m; r += G) {
                     if (r < i) continue;
-                    const double vr = (r == i) ? 1.0 : ai[r];
+                    const double air = ai[r];
+                    double x[4];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        if (live[t]) p[t] = __fma_rn(vr, cp[t][r], p[t]);
+                    for (int t = 0; t < 4; ++t) x[t] = cp[t][r];
+                    const double vr = (r == i) ? 1.0 : air;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) p[t] = live[t] ? __fma_rn(vr, x[t], p[t]) : p[t];
                 }
                 Gr::sum_bcast_n(p);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) p[t] = mt * p[t];
                 for (int r = lane; r < m; r += G) {
                     if (r < i) continue;
-                    const double vr = (r == i) ? 1.0 : ai[r];
+                    const double air = ai[r];
+                    double x[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) x[t] = cp[t][r];
+                    const double vr = (r == i) ? 1.0 : air;
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        if (live[t]) cp[t][r] = __fma_rn(vr, p[t], cp[t][r]);
+                        if (live[t]) cp[t][r] = __fma_rn(vr, p[t], x[t]);
                 }
             }
             __syncthreads();
@@ -1397,13 +1408,26 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
         }
     }
     double sum = 0.0;
-    if (lane == 0) {
-        for (int i = 0; i < m; ++i) {
-            double v = s.r[i];
-            sum = __dadd_rn(sum, __dmul_rn(v, v));
+    if constexpr (G == kWave) {
+        /* the same sequential sum: each square formed on its row's lane,
+         * the terms taken in row order by readlane (no LDS round trip per
+         * row on one lane) */
+        for (int i0 = 0; i0 < m; i0 += kWave) {
+            const int i = i0 + lane;
+            const double v = i < m ? s.r[i] : 0.0;
+            const double sq = __dmul_rn(v, v);
+            const int cnt = m - i0 < kWave ? m - i0 : kWave;
+            for (int l = 0; l < cnt; ++l) sum = __dadd_rn(sum, Gr::lane_value(sq, l));
         }
+    } else {
+        if (lane == 0) {
+            for (int i = 0; i < m; ++i) {
+                double v = s.r[i];
+                sum = __dadd_rn(sum, __dmul_rn(v, v));
+            }
+        }
+        sum = Gr::bcast0(sum);
     }
-    sum = Gr::bcast0(sum);
     return sqrt(sum);
 }
 

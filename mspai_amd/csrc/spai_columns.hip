@@ -383,10 +383,11 @@ __device__ __forceinline__ int make_rhs(const BuildArgs& a, const WaveScratch& s
 
 /* Approximative_Solution (src/Spai_Unrestrained.imp:135-195): QR, Q^T b,
  * dtrtrs.  On exit b[0..n) holds the solution.  Returns status bits. */
-template <int G>
+template <int G, int kC = 4>
 __device__ __forceinline__ int ls_solve(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
     using Gr = Group<G>;
+    static_assert(kC % 4 == 0, "columns per group");
     const int lane = Gr::lane();
     const int ld = L.capI;
     int k = m < n ? m : n;
@@ -413,49 +414,51 @@ __device__ __forceinline__ int ls_solve(const ScratchLayout& L, const WaveScratc
         }
         if (lane == 0) s.tau[i] = tau;
         if (tau != 0.0) {
-            /* H_i on columns i+1..n-1 and b, four at a time: each column's
+            /* H_i on columns i+1..n-1 and b, kC at a time: each column's
              * dot product and update are apply_reflector's (rows of a lane in
-             * order, fused; the 64-lane tree), but the four reductions share
-             * their lane moves and the loads of four columns are in flight
+             * order, fused; the 64-lane tree), but the reductions share
+             * their lane moves and the loads of kC columns are in flight
              * together; one barrier per reflector */
             const int nc = n - i;  // columns i+1..n-1, then b
             const double mt = -tau;
-            for (int c0 = 0; c0 < nc; c0 += 4) {
-                double* cp[4];
-                bool live[4];
+            for (int c0 = 0; c0 < nc; c0 += kC) {
+                double* cp[kC];
+                bool live[kC];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
+                for (int t = 0; t < kC; ++t) {
                     const int c = c0 + t;
                     live[t] = c < nc;
                     cp[t] = c < nc - 1 ? s.aqr + (i + 1 + c) * ld : s.b;
                 }
-                /* a row's five loads are issued together, unconditionally
+                /* a row's kC + 1 loads are issued together, unconditionally
                  * (a dead slot reads b, in bounds, and adds or stores
                  * nothing): behind per-slot branches each load waited for
                  * the one before it */
-                double p[4] = {0.0, 0.0, 0.0, 0.0};
+                double p[kC];
+#pragma unroll
+                for (int t = 0; t < kC; ++t) p[t] = 0.0;
                 for (int r = lane; r < m; r += G) {
                     if (r < i) continue;
                     const double air = ai[r];
-                    double x[4];
+                    double x[kC];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) x[t] = cp[t][r];
+                    for (int t = 0; t < kC; ++t) x[t] = cp[t][r];
                     const double vr = (r == i) ? 1.0 : air;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) p[t] = live[t] ? __fma_rn(vr, x[t], p[t]) : p[t];
+                    for (int t = 0; t < kC; ++t) p[t] = live[t] ? __fma_rn(vr, x[t], p[t]) : p[t];
                 }
                 Gr::sum_bcast_n(p);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) p[t] = mt * p[t];
+                for (int t = 0; t < kC; ++t) p[t] = mt * p[t];
                 for (int r = lane; r < m; r += G) {
                     if (r < i) continue;
                     const double air = ai[r];
-                    double x[4];
+                    double x[kC];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) x[t] = cp[t][r];
+                    for (int t = 0; t < kC; ++t) x[t] = cp[t][r];
                     const double vr = (r == i) ? 1.0 : air;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    for (int t = 0; t < kC; ++t)
                         if (live[t]) cp[t][r] = __fma_rn(vr, p[t], x[t]);
                 }
             }
@@ -1569,21 +1572,34 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
             lmax = max(lmax, len[k]);
         }
         for (int base = 0; base < lmax; base += kR) {
+            /* with several candidates per lane (the wide LDS kernels, a few
+             * waves per CU, latency-bound) the values load with the row
+             * indices instead of waiting for the lookups, one memory round
+             * trip instead of two; elsewhere only the hits load a value
+             * (fewer bytes for the many resident waves) */
+            constexpr bool kEager = kB > 1;
             int pos[kB][kR];
+            double v[kB][kR];
 #pragma unroll
             for (int k = 0; k < kB; ++k)
 #pragma unroll
-                for (int q = 0; q < kR; ++q) pos[k][q] = base + q < len[k] ? a.A.rowind[e0[k] + base + q] : -1;
+                for (int q = 0; q < kR; ++q) {
+                    const bool in = base + q < len[k];
+                    pos[k][q] = in ? a.A.rowind[e0[k] + base + q] : -1;
+                    v[k][q] = (kEager && in) ? a.A.val[e0[k] + base + q] : 0.0;
+                }
 #pragma unroll
             for (int k = 0; k < kB; ++k)
 #pragma unroll
                 for (int q = 0; q < kR; ++q)
                     if (pos[k][q] >= 0) pos[k][q] = row_position(h, s.I, m, pos[k][q]);
-            double v[kB][kR];
+            if constexpr (!kEager) {
 #pragma unroll
-            for (int k = 0; k < kB; ++k)
+                for (int k = 0; k < kB; ++k)
 #pragma unroll
-                for (int q = 0; q < kR; ++q) v[k][q] = pos[k][q] >= 0 ? a.A.val[e0[k] + base + q] : 0.0;
+                    for (int q = 0; q < kR; ++q)
+                        if (pos[k][q] >= 0) v[k][q] = a.A.val[e0[k] + base + q];
+            }
 #pragma unroll
             for (int k = 0; k < kB; ++k)
 #pragma unroll
@@ -2391,7 +2407,9 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         } else if (a.cache_slots > 0) {
             st |= ls_solve_cached<G>(a, L, s, m, n);
         } else {
-            st |= ls_solve<G>(L, s, m, n);
+            /* eight columns per pass in the 64-lane LDS bins (registers to
+             * spare at their few waves per CU) */
+            st |= ls_solve<G, (G == kWave && !kTies) ? 8 : 4>(L, s, m, n);
         }
         ++solves;
         count_solve<G>(a, s, m, n, wk);

@@ -63,6 +63,24 @@ __global__ void column_norms_kernel(DevCsc A, double* ajn)
     ajn[j] = sqrt(s);
 }
 
+/* C's values in the order of its row pattern (which holds exactly C's
+ * entries, sorted per row): each stored entry finds its column in its row. */
+__global__ void row_values_kernel(DevCsc A, const int* rp_ptr, const int* rp_ind, double* rp_val)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.ncols) return;
+    for (int e = A.colptr[j]; e < A.colptr[j + 1]; ++e) {
+        const int i = A.rowind[e];
+        int lo = rp_ptr[i], hi = rp_ptr[i + 1];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (rp_ind[mid] < j) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < rp_ptr[i + 1] && rp_ind[lo] == j) rp_val[lo] = A.val[e];
+    }
+}
+
 /* Sum the per-column work counters (one block, fixed order). */
 __global__ void work_sum_kernel(const double* work, int n, double* out)
 {
@@ -117,6 +135,8 @@ struct mspai_ctx {
     double* d_val = nullptr;
     int *d_rp_ptr = nullptr, *d_rp_ind = nullptr;
     double* d_ajn = nullptr;
+    double* d_rp_val = nullptr;  // C's values in row-pattern order (filled with the column norms)
+    bool rp_match = false;       // the row pattern is C's own (then d_rp_val is allocated)
     // start pattern
     int *d_p_ptr = nullptr, *d_p_ind = nullptr;
     int plen_max = 0;
@@ -339,6 +359,7 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_rp_ptr);
     dfree(ctx->d_rp_ind);
     dfree(ctx->d_ajn);
+    dfree(ctx->d_rp_val);
     dfree(ctx->d_p_ptr);
     dfree(ctx->d_p_ind);
     dfree(ctx->d_b_ptr);
@@ -411,6 +432,26 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
         }
     }
     std::vector<int> rp, ri;
+    bool rp_match = true;  // a derived pattern is C's own
+    if (rowptr && colind) {
+        /* a given pattern is C's own when it holds exactly C's entries,
+         * sorted per row (then the scores may run over its rows) */
+        std::vector<int> cnt(nrows + 1, 0);
+        for (long long e = 0; e < nnz; ++e) cnt[rowind[e] + 1]++;
+        for (int i = 0; i < nrows && rp_match; ++i) rp_match = rowptr[i + 1] - rowptr[i] == cnt[i + 1];
+        if (rp_match && rowptr[0] == 0) {
+            std::vector<int> fill(nrows);
+            for (int i = 0; i < nrows; ++i) fill[i] = rowptr[i];
+            for (int j = 0; j < ncols && rp_match; ++j)
+                for (int e = colptr[j]; e < colptr[j + 1]; ++e)
+                    if (colind[fill[rowind[e]]++] != j) {
+                        rp_match = false;
+                        break;
+                    }
+        } else {
+            rp_match = false;
+        }
+    }
     if (!rowptr || !colind) {
         /* CSR pattern of C: the row_idcs of Compressed_Lines */
         rp.assign(nrows + 1, 0);
@@ -434,7 +475,13 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
     if ((rc = upload(ctx, ctx->d_rp_ptr, rowptr, nrows + 1))) return rc;
     if ((rc = upload(ctx, ctx->d_rp_ind, colind, nnz))) return rc;
     if ((rc = upload<double>(ctx, ctx->d_ajn, nullptr, ncols))) return rc;
+    if (rp_match && maxr <= kWave) {
+        if ((rc = upload<double>(ctx, ctx->d_rp_val, nullptr, nnz))) return rc;
+    } else {
+        dfree(ctx->d_rp_val);
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->rp_match = rp_match && maxr <= kWave;
     ctx->nrows = nrows;
     ctx->ncols = ncols;
     ctx->nnz = nnz;
@@ -1183,6 +1230,7 @@ static BuildArgs base_args(mspai_ctx* ctx)
     a.A = DevCsc{ctx->nrows, ctx->ncols, ctx->d_colptr, ctx->d_rowind, ctx->d_val};
     a.rp_ptr = ctx->d_rp_ptr;
     a.rp_ind = ctx->d_rp_ind;
+    a.rp_val = ctx->rp_match ? ctx->d_rp_val : nullptr;  // filled with the column norms, before any use
     a.B = DevCsc{ctx->nrows, ctx->ncols, ctx->d_b_ptr, ctx->d_b_ind, ctx->d_b_val};
     a.ajn = ctx->d_ajn;
     a.scratch = ctx->d_scratch;
@@ -1231,6 +1279,9 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     BuildArgs a = base_args(ctx);
     if (!ctx->ajn_ok) {  // once per set of values
         hipLaunchKernelGGL(column_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, a.A, ctx->d_ajn);
+        if (a.rp_val)
+            hipLaunchKernelGGL(row_values_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, a.A, a.rp_ptr,
+                               a.rp_ind, ctx->d_rp_val);
         HIPCHK(hipGetLastError());
         ctx->ajn_ok = true;
     }

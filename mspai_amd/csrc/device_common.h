@@ -83,6 +83,8 @@ struct BuildArgs {
     DevCsc A;               // C (A, or A with probing rows)
     const int* rp_ptr;      // row pattern of C (CSR)
     const int* rp_ind;
+    const double* rp_val;   // C's values in row-pattern order: candidate scores over the rows of I
+                            // (nullptr: over the candidates' columns)
     const int* p_ptr;       // start pattern (CSC)
     const int* p_ind;
     DevCsc B;               // target; B.colptr == nullptr means identity

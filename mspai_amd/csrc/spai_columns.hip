@@ -1434,6 +1434,113 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
     return sqrt(sum);
 }
 
+/* The candidate scores of augment with the numerators r^T a_j summed over
+ * the rows of I instead of over the columns of the candidates: row i of I
+ * adds r_i a_ij to the numerator of every candidate j of its row, the rows
+ * in increasing order, so each numerator takes its terms in the order of
+ * its column's stored entries restricted to I -- the column-wise sum, the
+ * same bits -- while only the entries of I's rows are read (a candidate
+ * column's entries outside I are never touched).  Whole waves only.
+ *   1. The raw list (the row patterns of I then of col, from the union) is
+ *      turned in place into candidate slots: rank of j in the bitmap of
+ *      candidates (word ranks in the words after the bitmap), -1 for j in J.
+ *   2. Numerators (in the score slots, zeroed): rows of I in order, eight
+ *      rows' slots and values loaded together, then each row's entries
+ *      (distinct slots) updated in turn.
+ *   3. rho_j from the numerator and ||a_j||, in its slot.
+ * `rp_val` holds C's values in row-pattern order; rows hold at most 64
+ * entries. */
+template <bool kMixed, int kA>
+__device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScratch& s, int m, int c, int off,
+                                               int vmin, int words, unsigned* bm, double rn, double rn2,
+                                               bool* rho_near0, double* acc4)
+{
+    using Gr = Group<kWave>;
+    constexpr int kD = 8;  // rows whose loads are in flight together
+    const int lane = Gr::lane();
+    unsigned* const pre = bm + words;  // candidates in the words before
+    int run = 0;
+    for (int w0 = 0; w0 < words; w0 += kWave) {
+        const int w = w0 + lane;
+        const int cnt = w < words ? __popc(bm[w]) : 0;
+        const int incl = Gr::scan_incl(cnt);
+        if (w < words) pre[w] = (unsigned)(run + incl - cnt);
+        run += Gr::bcast_last(incl);
+    }
+    __syncthreads();
+    for (int k = lane; k < off; k += kWave) {
+        const int d = s.sbuf[k] - vmin;
+        const unsigned bits = bm[d >> 5], bit = 1u << (d & 31);
+        s.sbuf[k] = (bits & bit) ? (int)pre[d >> 5] + __popc(bits & (bit - 1u)) : -1;
+    }
+    __syncthreads();
+    double* const num = s.rho;  // the word ranks are dead now
+    for (int t = lane; t < c; t += kWave) num[t] = 0.0;
+    __syncthreads();
+    run = 0;
+    for (int ip0 = 0; ip0 < m; ip0 += kWave) {
+        const int ip = ip0 + lane;
+        int len = 0, base = 0;
+        double rr = 0.0;
+        if (ip < m) {
+            const int i = s.I[ip];
+            base = a.rp_ptr[i];
+            len = a.rp_ptr[i + 1] - base;
+            rr = s.r[ip];
+        }
+        const int incl = Gr::scan_incl(len);
+        const int roff = run + incl - len;  // the row's entries in the raw list
+        run += Gr::bcast_last(incl);
+        const int rows = m - ip0 < kWave ? m - ip0 : kWave;
+        for (int q0 = 0; q0 < rows; q0 += kD) {
+            int slot[kD];
+            double val[kD];
+#pragma unroll
+            for (int d = 0; d < kD; ++d) {
+                slot[d] = -1;
+                val[d] = 0.0;
+                const int q = q0 + d;
+                if (q < rows && lane < Gr::lane_value_i(len, q)) {
+                    slot[d] = s.sbuf[Gr::lane_value_i(roff, q) + lane];
+                    val[d] = a.rp_val[Gr::lane_value_i(base, q) + lane];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < kD; ++d) {
+                const int q = q0 + d;
+                if (q < rows) {
+                    const double rq = Gr::lane_value(rr, q);
+                    if (slot[d] >= 0) num[slot[d]] = __dadd_rn(num[slot[d]], __dmul_rn(rq, val[d]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int t0 = lane; t0 < c; t0 += kA * kWave) {
+        double ajn[kA];
+#pragma unroll
+        for (int k = 0; k < kA; ++k) {
+            const int t = t0 + k * kWave;
+            ajn[k] = t < c ? a.ajn[s.jt[t]] : 1.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kA; ++k) {
+            const int t = t0 + k * kWave;
+            if (t < c) {
+                const double nm = num[t];
+                const double n2 = __dmul_rn(nm, nm);
+                const double rho = __dsub_rn(rn2, __ddiv_rn(n2, __dmul_rn(ajn[k], ajn[k])));
+                if (kMixed) *rho_near0 |= rho <= kGuard * rn * rn;
+                s.rho[t] = sqrt(rho);
+                if (acc4) {
+                    const int j = s.jt[t];
+                    *acc4 += 16.0 + 12.0 * (a.A.colptr[j + 1] - a.A.colptr[j]);
+                }
+            }
+        }
+    }
+}
+
 /* Augment_Sparsity (src/Spai_Sub.imp:1010-1116) with Union_Nl_Sets and
  * Compute_Rhos.  Returns the new |J|, -1 when J~ is empty, or kOverflow.
  * Without kTies a tie of scores across the cut also returns kOverflow: the
@@ -1499,9 +1606,13 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
     int avail = 0;
     unsigned* const bm = bitmap_words<kTies>(L, s, &avail);
     int c = 0;
+    bool by_rows = false;
     if (off > 0 && ((vmax - vmin) >> 5) < avail) {
         c = bitmap_unique<G>(s.sbuf, off, vmin, vmax, s.J, n, s.jt, L.capU, bm);
         if (c == kOverflow) return kOverflow;
+        /* the raw list is intact and the bitmap leaves room for its word
+         * ranks: the numerators can run over the rows of I */
+        by_rows = !kTies && G == kWave && a.rp_val != nullptr && 2 * (((vmax - vmin) >> 5) + 1) <= avail;
     } else {
         int nu = group_sort_unique<G>(s.sbuf, off, s.sbuf);
         if (nu > L.capU) return kOverflow;
@@ -1532,9 +1643,13 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
      * indices of each per round, all in flight together; only the entries
      * whose row is in I (a small share) load their value.  Every numerator
      * still adds its column's entries in order. */
-    const RowHash h = row_hash_build<G>(s, L, m, a.A.nrows);
     bool rho_near0 = false;  // mixed path: rho^2 within rounding of 0 (its sign decides NaN or not)
     const double rn2 = __dmul_rn(rn, rn);
+    if (by_rows) {
+        scores_by_rows<kMixed, kA>(a, s, m, c, off, vmin, ((vmax - vmin) >> 5) + 1, bm, rn, rn2, &rho_near0,
+                                   wk ? &acc4 : nullptr);
+    } else {
+    const RowHash h = row_hash_build<G>(s, L, m, a.A.nrows);
     int2* const bounds = reinterpret_cast<int2*>(s.rho);
     for (int t0 = lane; t0 < c; t0 += kA * G) {
         int lo[kA], hi[kA];
@@ -1617,6 +1732,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
                 acc4 += 16.0 + 12.0 * len[k];
             }
         }
+    }
     }
     __syncthreads();
     if (kMixed && Gr::ballot(rho_near0) != 0ull) return kFallback;

@@ -1231,6 +1231,7 @@ static BuildArgs base_args(mspai_ctx* ctx)
     a.rp_ptr = ctx->d_rp_ptr;
     a.rp_ind = ctx->d_rp_ind;
     a.rp_val = ctx->rp_match ? ctx->d_rp_val : nullptr;  // filled with the column norms, before any use
+    a.rp_maxlen = ctx->maxrownnz;
     a.B = DevCsc{ctx->nrows, ctx->ncols, ctx->d_b_ptr, ctx->d_b_ind, ctx->d_b_val};
     a.ajn = ctx->d_ajn;
     a.scratch = ctx->d_scratch;

@@ -85,6 +85,7 @@ struct BuildArgs {
     const int* rp_ind;
     const double* rp_val;   // C's values in row-pattern order: candidate scores over the rows of I
                             // (nullptr: over the candidates' columns)
+    int rp_maxlen;          // longest row of the pattern (groups of fewer lanes score by columns)
     const int* p_ptr;       // start pattern (CSC)
     const int* p_ind;
     DevCsc B;               // target; B.colptr == nullptr means identity

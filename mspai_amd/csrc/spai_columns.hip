@@ -1440,7 +1440,7 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
  * in increasing order, so each numerator takes its terms in the order of
  * its column's stored entries restricted to I -- the column-wise sum, the
  * same bits -- while only the entries of I's rows are read (a candidate
- * column's entries outside I are never touched).  Whole waves only.
+ * column's entries outside I are never touched).
  *   1. The raw list (the row patterns of I then of col, from the union) is
  *      turned in place into candidate slots: rank of j in the bitmap of
  *      candidates (word ranks in the words after the bitmap), -1 for j in J.
@@ -1448,19 +1448,19 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
  *      rows' slots and values loaded together, then each row's entries
  *      (distinct slots) updated in turn.
  *   3. rho_j from the numerator and ||a_j||, in its slot.
- * `rp_val` holds C's values in row-pattern order; rows hold at most 64
+ * `rp_val` holds C's values in row-pattern order; rows hold at most G
  * entries. */
-template <bool kMixed, int kA>
+template <int G, bool kMixed, int kA>
 __device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScratch& s, int m, int c, int off,
                                                int vmin, int words, unsigned* bm, double rn, double rn2,
                                                bool* rho_near0, double* acc4)
 {
-    using Gr = Group<kWave>;
+    using Gr = Group<G>;
     constexpr int kD = 8;  // rows whose loads are in flight together
     const int lane = Gr::lane();
     unsigned* const pre = bm + words;  // candidates in the words before
     int run = 0;
-    for (int w0 = 0; w0 < words; w0 += kWave) {
+    for (int w0 = 0; w0 < words; w0 += G) {
         const int w = w0 + lane;
         const int cnt = w < words ? __popc(bm[w]) : 0;
         const int incl = Gr::scan_incl(cnt);
@@ -1468,17 +1468,17 @@ __device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScr
         run += Gr::bcast_last(incl);
     }
     __syncthreads();
-    for (int k = lane; k < off; k += kWave) {
+    for (int k = lane; k < off; k += G) {
         const int d = s.sbuf[k] - vmin;
         const unsigned bits = bm[d >> 5], bit = 1u << (d & 31);
         s.sbuf[k] = (bits & bit) ? (int)pre[d >> 5] + __popc(bits & (bit - 1u)) : -1;
     }
     __syncthreads();
     double* const num = s.rho;  // the word ranks are dead now
-    for (int t = lane; t < c; t += kWave) num[t] = 0.0;
+    for (int t = lane; t < c; t += G) num[t] = 0.0;
     __syncthreads();
     run = 0;
-    for (int ip0 = 0; ip0 < m; ip0 += kWave) {
+    for (int ip0 = 0; ip0 < m; ip0 += G) {
         const int ip = ip0 + lane;
         int len = 0, base = 0;
         double rr = 0.0;
@@ -1491,7 +1491,7 @@ __device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScr
         const int incl = Gr::scan_incl(len);
         const int roff = run + incl - len;  // the row's entries in the raw list
         run += Gr::bcast_last(incl);
-        const int rows = m - ip0 < kWave ? m - ip0 : kWave;
+        const int rows = m - ip0 < G ? m - ip0 : G;
         for (int q0 = 0; q0 < rows; q0 += kD) {
             int slot[kD];
             double val[kD];
@@ -1500,9 +1500,16 @@ __device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScr
                 slot[d] = -1;
                 val[d] = 0.0;
                 const int q = q0 + d;
-                if (q < rows && lane < Gr::lane_value_i(len, q)) {
-                    slot[d] = s.sbuf[Gr::lane_value_i(roff, q) + lane];
-                    val[d] = a.rp_val[Gr::lane_value_i(base, q) + lane];
+                if (q < rows) {
+                    /* the row's numbers from its lane while every lane of the
+                     * group is active (16-lane groups read them by permute,
+                     * which sees only active lanes) */
+                    const int ln = Gr::lane_value_i(len, q), ro = Gr::lane_value_i(roff, q);
+                    const int bs = Gr::lane_value_i(base, q);
+                    if (lane < ln) {
+                        slot[d] = s.sbuf[ro + lane];
+                        val[d] = a.rp_val[bs + lane];
+                    }
                 }
             }
 #pragma unroll
@@ -1516,16 +1523,16 @@ __device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScr
         }
     }
     __syncthreads();
-    for (int t0 = lane; t0 < c; t0 += kA * kWave) {
+    for (int t0 = lane; t0 < c; t0 += kA * G) {
         double ajn[kA];
 #pragma unroll
         for (int k = 0; k < kA; ++k) {
-            const int t = t0 + k * kWave;
+            const int t = t0 + k * G;
             ajn[k] = t < c ? a.ajn[s.jt[t]] : 1.0;
         }
 #pragma unroll
         for (int k = 0; k < kA; ++k) {
-            const int t = t0 + k * kWave;
+            const int t = t0 + k * G;
             if (t < c) {
                 const double nm = num[t];
                 const double n2 = __dmul_rn(nm, nm);
@@ -1612,7 +1619,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         if (c == kOverflow) return kOverflow;
         /* the raw list is intact and the bitmap leaves room for its word
          * ranks: the numerators can run over the rows of I */
-        by_rows = !kTies && G == kWave && a.rp_val != nullptr && 2 * (((vmax - vmin) >> 5) + 1) <= avail;
+        by_rows = !kTies && a.rp_val != nullptr && a.rp_maxlen <= G && 2 * (((vmax - vmin) >> 5) + 1) <= avail;
     } else {
         int nu = group_sort_unique<G>(s.sbuf, off, s.sbuf);
         if (nu > L.capU) return kOverflow;
@@ -1646,7 +1653,7 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
     bool rho_near0 = false;  // mixed path: rho^2 within rounding of 0 (its sign decides NaN or not)
     const double rn2 = __dmul_rn(rn, rn);
     if (by_rows) {
-        scores_by_rows<kMixed, kA>(a, s, m, c, off, vmin, ((vmax - vmin) >> 5) + 1, bm, rn, rn2, &rho_near0,
+        scores_by_rows<G, kMixed, kA>(a, s, m, c, off, vmin, ((vmax - vmin) >> 5) + 1, bm, rn, rn2, &rho_near0,
                                    wk ? &acc4 : nullptr);
     } else {
     const RowHash h = row_hash_build<G>(s, L, m, a.A.nrows);

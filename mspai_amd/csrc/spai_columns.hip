@@ -1398,14 +1398,61 @@ __device__ __forceinline__ double residual_norm(const BuildArgs& a, const Scratc
     } else {
         for (int i = lane; i < m; i += G) s.r[i] = -s.b0[i];
         __syncthreads();
-        for (int jj = 0; jj < n; ++jj) {
-            double xj = s.b[jj];
-            if (xj == 0.0) continue;
-            int j = s.J[jj];
-            int e0 = a.A.colptr[j], cnt = a.A.colptr[j + 1] - e0;
-            for (int e = lane; e < cnt; e += G) {
-                int pos = row_position(h, s.I, m, a.A.rowind[e0 + e]);
-                s.r[pos] = __fma_rn(xj, a.A.val[e0 + e], s.r[pos]);
+        /* the columns' bounds and x_j one per lane; then kD columns' entries
+         * loaded and their rows found together, the updates applied column
+         * by column (a row takes its terms in order of j, as before) */
+        constexpr int kD = 4;
+        for (int j0 = 0; j0 < n; j0 += G) {
+            const int jj = j0 + lane;
+            int e0 = 0, cnt = 0;
+            double xj = 0.0;
+            if (jj < n) {
+                xj = s.b[jj];
+                const int j = s.J[jj];
+                e0 = a.A.colptr[j];
+                cnt = xj != 0.0 ? a.A.colptr[j + 1] - e0 : 0;
+            }
+            const int cols = n - j0 < G ? n - j0 : G;
+            if (Gr::max_int(cnt) > G) {
+                /* a column longer than the group: one column at a time */
+                for (int q = 0; q < cols; ++q) {
+                    const int c = Gr::lane_value_i(cnt, q), b = Gr::lane_value_i(e0, q);
+                    const double x = Gr::lane_value(xj, q);
+                    for (int e = lane; e < c; e += G) {
+                        const int pos = row_position(h, s.I, m, a.A.rowind[b + e]);
+                        s.r[pos] = __fma_rn(x, a.A.val[b + e], s.r[pos]);
+                    }
+                    __syncthreads();
+                }
+                continue;
+            }
+            for (int q0 = 0; q0 < cols; q0 += kD) {
+                int pos[kD];
+                double v[kD];
+#pragma unroll
+                for (int d = 0; d < kD; ++d) {
+                    pos[d] = -1;
+                    v[d] = 0.0;
+                    const int q = q0 + d;
+                    if (q < cols) {
+                        const int c = Gr::lane_value_i(cnt, q), b = Gr::lane_value_i(e0, q);
+                        if (lane < c) {
+                            pos[d] = a.A.rowind[b + lane];
+                            v[d] = a.A.val[b + lane];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < kD; ++d)
+                    if (pos[d] >= 0) pos[d] = row_position(h, s.I, m, pos[d]);
+#pragma unroll
+                for (int d = 0; d < kD; ++d) {
+                    const int q = q0 + d;
+                    if (q < cols) {
+                        const double x = Gr::lane_value(xj, q);
+                        if (pos[d] >= 0) s.r[pos[d]] = __fma_rn(x, v[d], s.r[pos[d]]);
+                    }
+                }
             }
             __syncthreads();
         }
@@ -2449,7 +2496,7 @@ __device__ __forceinline__ void save_column_state(const BuildArgs& a, const Wave
     __syncthreads();
 }
 
-template <int G, bool kMixed, int NMAX, bool kTies>
+template <int G, bool kMixed, int NMAX, bool kTies, bool kTail = false>
 __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int col)
 {
@@ -2531,15 +2578,19 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
             st |= ls_solve_cached<G>(a, L, s, m, n);
         } else {
             /* eight columns per pass in the 64-lane LDS bins (registers to
-             * spare at their few waves per CU) */
-            st |= ls_solve<G, (G == kWave && !kTies) ? 8 : 4>(L, s, m, n);
+             * spare at their few waves per CU), sixteen in the widest bins
+             * (one wave per SIMD) */
+            st |= ls_solve<G, (G == kWave && !kTies) ? (kTail ? 16 : 8) : 4>(L, s, m, n);
         }
         ++solves;
         count_solve<G>(a, s, m, n, wk);
         /* the generic kernel's copy of A(I,J) gives the dense residual after
-         * any QR (the same bits as the sparse one) */
+         * any QR (the same bits as the sparse one); the size bins' sparse
+         * residual finds its rows through a hash in the raw-list region
+         * (dead after the LDS QR) */
         const double* Ad = (kTies && !dense) ? s.a0 : nullptr;
-        rn = residual_norm<G>(a, L, s, m, n, dense || Ad, hI, Ad);
+        const RowHash hR = (!kTies && !dense) ? row_hash_build<G>(s, L, m, a.A.nrows) : hI;
+        rn = residual_norm<G>(a, L, s, m, n, dense || Ad, hR, Ad);
         if (kMixed && !(fabs(rn - a.epsilon) > kGuard * a.epsilon)) {
             redo = true;
             moved = step;
@@ -2676,7 +2727,7 @@ __global__ __launch_bounds__(64, kWaves > 0 ? kWaves
             if (q < nq) {
                 col = queue[q];
                 if (kMode < 2)
-                    outcome = solve_column<G, kMode == 1, NMAX, !kLds>(a, L, s, col);
+                    outcome = solve_column<G, kMode == 1, NMAX, !kLds, kWaves == 1>(a, L, s, col);
                 else if (kMode < 4)
                     outcome = backward_column<G, kMode == 3, NMAX, !kLds>(a, L, s, col);
                 else
@@ -2821,6 +2872,9 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode, i
         /* |J| <= 16: the row-group QR with 4 column slots (fewer registers,
          * less spilled at 3 waves per SIMD) */
         if (regs && L.capJ <= 16) return &spai_column_kernel<64, true, 0, 16, 3>;
+        /* a bin whose LDS holds one wave per SIMD at most: the kernel
+         * compiled for that (sixteen columns per QR pass) */
+        if (!regs && L.per_wave > (160u << 10) / 5) return &spai_column_kernel<64, true, 0, 0, 1>;
         return regs ? &spai_column_kernel<64, true, 0, 32> : &spai_column_kernel<64, true, 0, 0>;
     case 1:
         if (group == 16) return regs ? &spai_column_kernel<16, true, 1, 8> : &spai_column_kernel<16, true, 1, 0>;

@@ -214,6 +214,33 @@ __device__ __forceinline__ int bitmap_unique(const int* src, int total, int lo, 
     return c;
 }
 
+/* Word ranks of a bitmap: pre[w] = set bits in the words before w, stored
+ * after the bitmap (words more words).  The rank of a set bit d is then
+ * pre[d >> 5] + popc(bits below d in its word), its position in the sorted
+ * list bitmap_unique read out. */
+template <int G>
+__device__ __forceinline__ void bitmap_ranks(unsigned* bm, int words)
+{
+    using Gr = Group<G>;
+    const int lane = Gr::lane();
+    unsigned* const pre = bm + words;
+    int run = 0;
+    for (int w0 = 0; w0 < words; w0 += G) {
+        const int w = w0 + lane;
+        const int cnt = w < words ? __popc(bm[w]) : 0;
+        const int incl = Gr::scan_incl(cnt);
+        if (w < words) pre[w] = (unsigned)(run + incl - cnt);
+        run += Gr::bcast_last(incl);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int bitmap_rank(const unsigned* bm, int words, int d)
+{
+    const unsigned bits = bm[d >> 5], bit = 1u << (d & 31);
+    return (bits & bit) ? (int)bm[words + (d >> 5)] + __popc(bits & (bit - 1u)) : -1;
+}
+
 /* LDS words for bitmap_unique: in a size bin the score region (free outside
  * the scoring, and aliased with A(I,J) only while A(I,J) is dead), in the
  * generic kernel the LDS hash region (its working set is in HBM, where
@@ -234,7 +261,8 @@ __device__ __forceinline__ unsigned* bitmap_words(const ScratchLayout& L, const 
  * or kOverflow when the raw list or I exceeds the layout.  Leaves the column
  * segments of J for gather_ahat.  kHbm: the working set is in HBM. */
 template <int G, bool kHbm>
-__device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int n)
+__device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s, int n,
+                                       bool* packed = nullptr)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
@@ -253,8 +281,21 @@ __device__ __forceinline__ int build_I(const BuildArgs& a, const ScratchLayout& 
     vmax = Gr::max_int(vmax);
     int avail = 0;
     unsigned* const bm = bitmap_words<kHbm>(L, s, &avail);
-    if (total > 0 && ((vmax - vmin) >> 5) < avail)
-        return bitmap_unique<G>(s.sbuf, total, vmin, vmax, nullptr, 0, s.I, L.capI, bm);
+    if (total > 0 && ((vmax - vmin) >> 5) < avail) {
+        const int m = bitmap_unique<G>(s.sbuf, total, vmin, vmax, nullptr, 0, s.I, L.capI, bm);
+        const int words = ((vmax - vmin) >> 5) + 1;
+        if (packed && m != kOverflow && 2 * words <= avail && L.capI < 0x10000 && L.capJ < 0x8000) {
+            /* every entry's position in I while the bitmap is at hand,
+             * packed below its column slot: the gather then needs neither
+             * the row index nor a search of I */
+            bitmap_ranks<G>(bm, words);
+            for (int e = lane; e < total; e += G)
+                s.sid[e] = (s.sid[e] << 16) | bitmap_rank(bm, words, s.sbuf[e] - vmin);
+            __syncthreads();
+            *packed = true;
+        }
+        return m;
+    }
     int m = group_sort_unique<G>(s.sbuf, total, s.sbuf);
     if (m > L.capI) return kOverflow;
     for (int i = lane; i < m; i += G) s.I[i] = s.sbuf[i];
@@ -323,7 +364,8 @@ __device__ __forceinline__ int row_position(const RowHash& h, const int* I, int 
  * Uses the column segments build_I left for this J. */
 template <int G, bool kMixed, bool kCopy = false>
 __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
-                                            int m, int n, const RowHash& h = RowHash{nullptr, 0, 0})
+                                            int m, int n, const RowHash& h = RowHash{nullptr, 0, 0},
+                                            bool packed = false)
 {
     using Gr = Group<G>;
     const int lane = Gr::lane();
@@ -338,9 +380,11 @@ __device__ __forceinline__ void gather_ahat(const BuildArgs& a, const ScratchLay
     __syncthreads();
     const int total = s.segoff[n];
     for (int e = lane; e < total; e += G) {
-        const int t = s.sid[e];
+        const int pk = s.sid[e];
+        const int t = packed ? pk >> 16 : pk;
         const int src = s.segs[t] + e - s.segoff[t];
-        const int pos = row_position(h, s.I, m, a.A.rowind[src]);  // every row of J's columns is in I
+        /* every row of J's columns is in I */
+        const int pos = packed ? (pk & 0xffff) : row_position(h, s.I, m, a.A.rowind[src]);
         const double v = a.A.val[src];
         s.aqr[t * ld + pos] = v;
         if (a0) a0[t * ld + pos] = v;
@@ -1505,26 +1549,13 @@ __device__ __forceinline__ void scores_by_rows(const BuildArgs& a, const WaveScr
     using Gr = Group<G>;
     constexpr int kD = 8;  // rows whose loads are in flight together
     const int lane = Gr::lane();
-    unsigned* const pre = bm + words;  // candidates in the words before
+    bitmap_ranks<G>(bm, words);
+    for (int k = lane; k < off; k += G) s.sbuf[k] = bitmap_rank(bm, words, s.sbuf[k] - vmin);
+    __syncthreads();
     int run = 0;
-    for (int w0 = 0; w0 < words; w0 += G) {
-        const int w = w0 + lane;
-        const int cnt = w < words ? __popc(bm[w]) : 0;
-        const int incl = Gr::scan_incl(cnt);
-        if (w < words) pre[w] = (unsigned)(run + incl - cnt);
-        run += Gr::bcast_last(incl);
-    }
-    __syncthreads();
-    for (int k = lane; k < off; k += G) {
-        const int d = s.sbuf[k] - vmin;
-        const unsigned bits = bm[d >> 5], bit = 1u << (d & 31);
-        s.sbuf[k] = (bits & bit) ? (int)pre[d >> 5] + __popc(bits & (bit - 1u)) : -1;
-    }
-    __syncthreads();
     double* const num = s.rho;  // the word ranks are dead now
     for (int t = lane; t < c; t += G) num[t] = 0.0;
     __syncthreads();
-    run = 0;
     for (int ip0 = 0; ip0 < m; ip0 += G) {
         const int ip = ip0 + lane;
         int len = 0, base = 0;
@@ -2549,14 +2580,18 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         near0[1] = near[1];
         near0[2] = near[2];
         if (wk && lane < kNumWork) wk0 = wk[lane];
-        int m = build_I<G, kTies>(a, L, s, n);
+        /* the FP64 bins pack each entry's position in I for the gather (in
+         * the mixed bulk kernel it measured slower: 72 B of spill per lane
+         * instead of 48) */
+        bool packed = false;
+        int m = build_I<G, kTies>(a, L, s, n, (kTies || kMixed) ? nullptr : &packed);
         if (m == kOverflow) {
             moved = step;
             break;
         }
         /* the generic kernel (I in HBM) finds rows through the LDS hash */
         const RowHash hI = kTies ? row_hash_build<G>(s, L, m, a.A.nrows) : RowHash{nullptr, 0, 0};
-        gather_ahat<G, kMixed && NMAX == 0, kTies>(a, L, s, m, n, hI);
+        gather_ahat<G, kMixed && NMAX == 0, kTies>(a, L, s, m, n, hI, packed);
         st |= make_rhs<G>(a, s, m, col);
         bool dense = kMixed;  // the mixed QR leaves the FP64 A(I,J) intact
         if (kMixed) {

@@ -314,6 +314,52 @@ def test_every_bin_table_bitwise(oracle, gpu):
         b.close()
 
 
+def _widen_rows(prob, rows, width, seed):
+    """prob with each of `rows` given `width` more entries (small values in
+    the next columns, wrapping), so that the row pattern has rows longer
+    than a lane group."""
+    rng = np.random.default_rng(seed)
+    n = prob.n
+    A = prob.A.tocoo()
+    r = [A.row]
+    c = [A.col]
+    v = [A.data]
+    for i in rows:
+        cols = (i + 1 + np.arange(width)) % n
+        r.append(np.full(width, i))
+        c.append(cols)
+        v.append(rng.uniform(-0.05, 0.05, width))
+    W = sp.coo_matrix((np.concatenate(v), (np.concatenate(r), np.concatenate(c))), shape=A.shape).tocsc()
+    W.sum_duplicates()
+    W.sort_indices()
+    return gen.Problem(prob.name + "_wide_rows", W, prob.P, prob.B, epsilon=prob.epsilon, maxnew=prob.maxnew,
+                       max_impr_steps=prob.max_impr_steps)
+
+
+@pytest.mark.parametrize("width", [40, 80])
+def test_long_rows_bitwise(oracle, gpu, width):
+    """Candidate scores over the rows of I need every row of the pattern to
+    fit the lane group (and, for any group, at most 64 entries); rows longer
+    than that send the bins (40: the 16- and 32-lane ones; 80: all) to the
+    column-wise scores. Either way the oracle's bits, on tables with 16-,
+    32- and 64-lane bins."""
+    prob = _widen_rows(gen.config2(32), rows=[5, 300, 301, 777], width=width, seed=width)
+    assert np.diff(prob.A.tocsr().indptr).max() > width
+    ref = oracle.spai(prob, threads=THREADS)
+    b = gpu.Builder(0)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        opts = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps)
+        for t in (0, 1, 4, 12):
+            name = b.set_bin_table(t, precision=0)
+            print(f"[width {width} table {t} {name}]", flush=True)
+            b.build(binning=2, counters=1, **opts)
+            assert_bitwise(oracle, prob, ref, b.download(), b.column_info())
+    finally:
+        b.close()
+
+
 def test_build_stats_timings(gpu):
     """The build's event timings nest: the first size bin's launch lies inside
     the column-kernel window, which lies inside the whole build (FP64 and

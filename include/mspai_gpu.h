@@ -85,6 +85,10 @@ const char* mspai_last_error(const mspai_ctx* ctx);
  * the host).  Replaces Compressed_Lines (src/Compressed_Lines.h:52-101):
  * col_idcs/A/len_cols become rowind/val/colptr, row_idcs/len_rows become
  * colind/rowptr.  nrows > ncols holds probing rows (src/Read_mm_Matrix.cpp:532-667).
+ * When the row pattern holds exactly C's entries (always so when derived)
+ * and no row has more than 64, the candidate scores run over the rows of I
+ * with C's values in row order (nnz doubles of device memory); otherwise
+ * they run column-wise, with the same results.
  */
 int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int* colptr, const int* rowind,
                      const double* val, const int* rowptr, const int* colind);

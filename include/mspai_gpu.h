@@ -187,7 +187,7 @@ int mspai_backward(mspai_ctx* ctx, const mspai_opts* opts, const double* d_gM, d
  * and keeps the fastest for later builds with opts->binning == 2.  Results do
  * not depend on the table.  A new mspai_upload_csc clears the choice.
  */
-#define MSPAI_MAX_TUNE 24
+#define MSPAI_MAX_TUNE 32
 typedef struct {
     int ncandidates;
     int chosen;

@@ -49,7 +49,7 @@ class Stats(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("ncandidates", C.c_int), ("chosen", C.c_int), ("chosen_name", C.c_char * 32),
-                ("candidate_ms", C.c_double * 24)]
+                ("candidate_ms", C.c_double * 32)]
 
 
 _lib = None

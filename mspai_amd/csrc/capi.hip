@@ -919,6 +919,12 @@ static const BinTable kTables[] = {
      {{64, 16, 64, 512, 1024, 0}, {64, 24, 96, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
     {"g64x16-g64x20i112-g64x384/m1", 3,
      {{64, 16, 64, 512, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
+    /* (u) the bulk bin with room for 384 candidates instead of 512: about
+     * 1.5 KB less LDS per wave, one more resident wave per CU */
+    {"g64x16u-g64x20i112-g64x384", 3,
+     {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    {"g64x16u-g64x20i112-g64x384/m1", 3,
+     {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 

@@ -1266,23 +1266,42 @@ __device__ __forceinline__ double rowgroup_sum(const double (&p)[4])
     return v;
 }
 
-template <bool kROut, int NS = 8>
+/* The in-lane part of a 64-lane-order sum for the row-group layout with RT
+ * rows per lane: virtual lane v = q + 16u (u = 0..3) owns rows v and, when
+ * RT = 8, v + 64 (slot u + 4), which the order adds to the lane's chain
+ * first; f(t) gives slot t's term and `fma2(t, acc)` continues the chain. */
+template <int RT, class F, class F2>
+__device__ __forceinline__ double rowgroup_sum_rt(F first, F2 next)
+{
+    double p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        p[u] = first(u);
+        if constexpr (RT == 8) p[u] = next(u + 4, p[u]);
+    }
+    return rowgroup_sum(p);
+}
+
+/* NS column slots per lane (|J| <= 4 NS); slot NS holds b.  RT rows per
+ * lane and slot: 4 (|I| <= 64) or 8 (|I| <= 128: rows q + 16 t, t = 0..7;
+ * row r + 64 continues the chain of row r, as in the 64-lane order). */
+template <bool kROut, int NS = 8, int RT = 4>
 __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const WaveScratch& s, int m, int n)
 {
-    /* NS column slots per lane (|J| <= 4 NS); slot NS holds b */
+    static_assert(RT == 4 || RT == 8, "rows per lane");
     const int lane = lane_id();
     const int c = lane >> 4, q = lane & 15;
     const int ld = L.capI;
-    double a[NS + 1][4];
+    double a[NS + 1][RT];
 #pragma unroll
     for (int sl = 0; sl < NS; ++sl)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < RT; ++t) {
             const int j = c + 4 * sl, r = q + 16 * t;
             a[sl][t] = (r < m && j < n) ? s.aqr[j * ld + r] : 0.0;
         }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < RT; ++t) {
         const int r = q + 16 * t;
         a[NS][t] = (c == 0 && r < m) ? s.b[r] : 0.0;
     }
@@ -1297,16 +1316,20 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
             if (i >= k) break;
             const int ti = i >> 4, qi = i & 15;
             /* column i sits in row group ci, slot si */
-            double p[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int r = q + 16 * t;
-                p[t] = r > i ? __fma_rn(a[si][t], a[si][t], 0.0) : 0.0;
-            }
-            const double xn2 = Group<64>::lane_value(rowgroup_sum(p), 16 * ci);
+            const double xn2 = Group<64>::lane_value(
+                rowgroup_sum_rt<RT>(
+                    [&](int t) {
+                        const int r = q + 16 * t;
+                        return r > i ? __fma_rn(a[si][t], a[si][t], 0.0) : 0.0;
+                    },
+                    [&](int t, double acc) {
+                        const int r = q + 16 * t;
+                        return (r > i && r < m) ? __fma_rn(a[si][t], a[si][t], acc) : acc;
+                    }),
+                16 * ci);
             double aii_l = a[si][0];
 #pragma unroll
-            for (int t = 1; t < 4; ++t)
+            for (int t = 1; t < RT; ++t)
                 if (t == ti) aii_l = a[si][t];
             const double alpha = Group<64>::lane_value(aii_l, 16 * ci + qi);
             double rii = alpha;
@@ -1317,9 +1340,9 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
                 const double scal = 1.0 / (alpha - beta);
                 rii = beta;
                 /* v over this lane's rows, formed in row group ci, copied to all */
-                double v[4];
+                double v[RT];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
+                for (int t = 0; t < RT; ++t) {
                     const int r = q + 16 * t;
                     const double vl = r > i ? a[si][t] * scal : (r == i ? 1.0 : 0.0);
                     v[t] = rowgroup_bcast(vl, ci);
@@ -1333,13 +1356,15 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
                     if ((4 * sl + 3 > i && 4 * sl < n) || sl == NS) {
                         const int j = c + 4 * sl;
                         const bool live = sl == NS ? c == 0 : (j > i && j < n);
-                        double pp[4];
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) pp[t] = q + 16 * t >= i ? __fma_rn(v[t], a[sl][t], 0.0) : 0.0;
-                        const double w = rowgroup_sum(pp);
+                        const double w = rowgroup_sum_rt<RT>(
+                            [&](int t) { return q + 16 * t >= i ? __fma_rn(v[t], a[sl][t], 0.0) : 0.0; },
+                            [&](int t, double acc) {
+                                const int r = q + 16 * t;
+                                return (r >= i && r < m) ? __fma_rn(v[t], a[sl][t], acc) : acc;
+                            });
                         const double tw = mt * w;
 #pragma unroll
-                        for (int t = 0; t < 4; ++t)
+                        for (int t = 0; t < RT; ++t)
                             if (live && q + 16 * t >= i) a[sl][t] = __fma_rn(v[t], tw, a[sl][t]);
                     }
                 }
@@ -1351,7 +1376,7 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
 #pragma unroll
         for (int sl = 0; sl < NS; ++sl)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < RT; ++t) {
                 const int j = c + 4 * sl, r = q + 16 * t;
                 if (j < n && r <= j) s.rq[j * L.capJ + r] = a[sl][t];
             }
@@ -1363,7 +1388,7 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
     if (m < n) {
         st = COL_UNDERDET;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) a[NS][t] = 0.0;
+        for (int t = 0; t < RT; ++t) a[NS][t] = 0.0;
     } else if (__ballot(zero_diag) != 0ull) {
         st = COL_SINGULAR;
     } else {
@@ -1402,7 +1427,7 @@ __device__ __forceinline__ int ls_solve_rows64(const ScratchLayout& L, const Wav
         }
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < RT; ++t) {
         const int r = q + 16 * t;
         if (c == 0 && r < n) s.b[r] = a[NS][t];
     }
@@ -2531,6 +2556,9 @@ template <int G, bool kMixed, int NMAX, bool kTies, bool kTail = false>
 __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                             int col)
 {
+    /* NMAX 20 or 24 in a 64-lane bin: the row-group register QR with eight
+     * rows per lane (|I| <= 128, FP64 builds) */
+    constexpr bool kRows128 = G == kWave && !kMixed && (NMAX == 20 || NMAX == 24);
     using Gr = Group<G>;
     const int lane = Gr::lane();
     double* wk = a.work ? s.wsum : nullptr;  // summed in LDS, stored with the column
@@ -2600,11 +2628,13 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
                 moved = step;
                 break;
             }
-        } else if (NMAX > 0 && m <= G && n <= NMAX) {
+        } else if (NMAX > 0 && m <= (kRows128 ? 2 * G : G) && n <= NMAX) {
             /* the register QR costs less than a cache lookup (which reads the
              * stored block and factor back from HBM): -cs serves the other
              * solves; both give the same bits */
-            if constexpr (G == kWave && (NMAX == 32 || NMAX == 16))
+            if constexpr (kRows128)
+                st |= ls_solve_rows64<false, NMAX / 4, 8>(L, s, m, n);
+            else if constexpr (G == kWave && (NMAX == 32 || NMAX == 16))
                 st |= ls_solve_rows64<false, NMAX / 4>(L, s, m, n);
             else
                 st |= ls_solve_regs<G, (NMAX > 0 ? NMAX : 4)>(L, s, m, n);
@@ -2635,7 +2665,7 @@ __device__ __forceinline__ int solve_column(const BuildArgs& a, const ScratchLay
         if (rn <= a.epsilon || step == a.ns - 1) break;
         /* candidates in flight per lane: more where registers are plentiful
          * (64-lane LDS kernels without the register QR) */
-        constexpr bool kWide = G == 64 && NMAX == 0 && !kTies;
+        constexpr bool kWide = G == 64 && (NMAX == 0 || kRows128) && !kTies;
         int nn = augment<G, kMixed, kTies, kWide ? 2 : 1, (G == 64 && !kTies) ? 16 : 8, kWide ? 8 : 4>(
             a, L, s, col, m, n, rn, &st, wk, wk ? near : nullptr);
         if (nn == kOverflow) {
@@ -2907,6 +2937,10 @@ static ColumnKernel binned_kernel(const ScratchLayout& L, int group, int mode, i
         /* |J| <= 16: the row-group QR with 4 column slots (fewer registers,
          * less spilled at 3 waves per SIMD) */
         if (regs && L.capJ <= 16) return &spai_column_kernel<64, true, 0, 16, 3>;
+        /* 64 < |I| <= 128 and |J| <= 24: the register QR with eight rows
+         * per lane (A(I,J) stays in LDS for the dense residual) */
+        if (group == 64 && L.capI > 64 && L.capI <= 128 && L.capJ <= 24)
+            return L.capJ <= 20 ? &spai_column_kernel<64, true, 0, 20, 2> : &spai_column_kernel<64, true, 0, 24, 2>;
         /* a bin whose LDS holds one wave per SIMD at most: the kernel
          * compiled for that (sixteen columns per QR pass) */
         if (!regs && L.per_wave > (160u << 10) / 5) return &spai_column_kernel<64, true, 0, 0, 1>;

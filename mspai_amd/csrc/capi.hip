@@ -925,6 +925,18 @@ static const BinTable kTables[] = {
      {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
     {"g64x16u-g64x20i112-g64x384/m1", 3,
      {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
+    /* a second bin of |J| <= 24, |I| <= 128 (the eight-row register QR),
+     * alone or after the |J| <= 20, |I| <= 112 one */
+    {"g64x16u-g64x24i128-g64x384", 3,
+     {{64, 16, 64, 384, 1024, 0}, {64, 24, 128, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}},
+    {"g64x16u-g64x24i128-g64x384/m1", 3,
+     {{64, 16, 64, 384, 1024, 0}, {64, 24, 128, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 1},
+    {"g64x16u-x20i112-x24i128-x384", 4,
+     {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 24, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}},
+    {"g64x16u-x20i112-x24i128-x384/m1", 4,
+     {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 24, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}, 1},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 

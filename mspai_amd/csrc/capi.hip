@@ -772,8 +772,12 @@ static ScratchLayout make_layout(int capJ, size_t capI, size_t capU, size_t capS
     L.wsum = take(8 * 8);
     /* register bins convert A(I,J) to FP32 as they load it: aq32 holds R only */
     const bool regbin = register_bin(group, (int)capI, capJ);
-    L.ldq = regbin ? capJ : (int)capI;
-    if (needs & kNeedMixed) L.aq32 = take((size_t)L.ldq * capJ * 4);
+    /* the mixed builds' 64-lane LDS bins factor the Gram matrix instead of
+     * an FP32 copy of A(I,J) (not the adjoint, whose solve is the QR's) */
+    const bool gram = (needs & kNeedMixed) && !(needs & kNeedBackward) && group == 64 && !regbin;
+    L.gram = gram;
+    L.ldq = regbin ? capJ : (gram ? kGramLd : (int)capI);
+    if (needs & kNeedMixed) L.aq32 = take((size_t)L.ldq * (gram ? kGramLd : capJ) * 4);
     if (needs & kNeedBackward) L.rq = take((size_t)capJ * capJ * 8);
     if (needs & kNeedA0) {
         L.a0 = take(capI * capJ * 8);

@@ -25,6 +25,9 @@ enum ColStatus : int {
 };
 
 constexpr int kWave = 64;
+/* Leading dimension of the Gram matrix / R of the mixed 64-lane LDS bins
+ * (|J| <= 32; padded against bank conflicts). */
+constexpr int kGramLd = 33;
 constexpr int kNumWork = 8;
 constexpr int kLaunchStat = 13;  // per-launch totals: kNumWork counters, columns, 3 near-margin counts, nnz
 /* A selection decision of the FP64 path whose margin is at most kNearMargin
@@ -65,7 +68,10 @@ struct ScratchLayout {
     int capSid;    // flat entries of the columns of J (Get_I_Set's raw list, the A(I,J) gather)
     int capPairs;  // candidates on the std::sort tie path
     int ldq;       // leading dimension of aq32: capI (FP32 copy of A(I,J)), or
-                   // capJ in register bins, where aq32 holds only the FP32 R
+                   // capJ in register bins, where aq32 holds only the FP32 R, or
+                   // kGramLd with `gram`
+    int gram;      // mixed 64-lane LDS bin: R from the Gram matrix on the matrix
+                   // cores (aq32: kGramLd x kGramLd floats, the Gram rows, then R)
     int tie_in_main;        // pairs/stack in the main region (else in BuildArgs::tie_scratch)
     int max_rounds;         // most columns per group in one chunk of the work queue
     size_t tie_per_group;   // bytes of a group's tie region when not in the main one

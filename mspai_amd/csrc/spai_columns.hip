@@ -1003,6 +1003,88 @@ __device__ __forceinline__ int qr_regs_f32(const ScratchLayout& L, const WaveScr
     return 0;
 }
 
+/* The mixed factor of the 64-lane LDS bins: G = A^T A of the FP32-rounded
+ * A(I,J) on the matrix cores (v_mfma_f32_32x32x2f32: lane l supplies
+ * A(r0 + l/32, l%32), which is both the A^T and the A operand; two rows of
+ * A per instruction, two accumulators), written to LDS by its accumulator
+ * layout (lane l, register v: row 8(v/4) + 4(l/32) + v%4, column l%32),
+ * then an FP32 Cholesky G = R^T R with row p of G in the registers of lane
+ * p (rows of R broadcast through LDS, one pivot at a time).  R (upper,
+ * column-major, leading dimension kGramLd) is left in aq32 for the CSNE
+ * refinement.  kFallback when |J| > 32 (a 33rd column would need a second
+ * accumulator set and 33 registers per row: the kernel then spills), when
+ * a pivot is not positive, or when R's diagonal ratio is below kGramGuard
+ * (the refinement's rate is about kappa^2 u_32). */
+constexpr float kGramGuard = 3e-3f;
+constexpr int kGramSteps = 5;
+__device__ __forceinline__ int gram_chol(const ScratchLayout& L, const WaveScratch& s, int m, int n)
+{
+    typedef float f16v __attribute__((ext_vector_type(16)));
+    using Gr = Group<kWave>;
+    const int lane = Gr::lane();
+    if (n > 32 || n == 0 || m < n) return kFallback;
+    const int ci = lane & 31, kr = lane >> 5;
+    const bool live = ci < n;
+    const double* const col = s.aqr + (size_t)ci * L.capI;
+    f16v acc0 = {}, acc1 = {};
+    int r0 = 0;
+    for (; r0 + 4 <= m; r0 += 4) {
+        const float a0 = live ? (float)col[r0 + kr] : 0.f;
+        const float a1 = live ? (float)col[r0 + 2 + kr] : 0.f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, a0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, a1, acc1, 0, 0, 0);
+    }
+    for (; r0 < m; r0 += 2) {
+        const int r = r0 + kr;
+        const float a0 = (live && r < m) ? (float)col[r] : 0.f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, a0, acc0, 0, 0, 0);
+    }
+    float* const q = s.aq32;  // kGramLd x kGramLd
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int i = 8 * (v >> 2) + 4 * kr + (v & 3);
+        q[i * kGramLd + ci] = acc0[v] + acc1[v];  // row i, column ci of G
+    }
+    __syncthreads();
+    float g[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) g[c] = lane < n ? q[lane * kGramLd + c] : 0.f;
+    __syncthreads();
+    bool bad = false;
+    float dmax = 0.f, dmin = INFINITY;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        if (j >= n) break;
+        if (lane == j) {
+            /* row j of R: R_jj = sqrt(pivot), R_jc = G_jc / R_jj, to LDS
+             * (R_jc at q[c * kGramLd + j]) */
+            const float piv = g[j];
+            bad |= !(piv > 0.f);
+            const float d = sqrtf(piv > 0.f ? piv : 1.f);
+            const float inv = 1.f / d;
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                if (c >= j && c < n) q[c * kGramLd + j] = c == j ? d : g[c] * inv;
+            dmax = fmaxf(dmax, d);
+            dmin = fminf(dmin, d);
+        }
+        __syncthreads();
+        if (lane > j && lane < n) {
+            /* G_pc -= R_jp R_jc for c >= p (the upper part of row p) */
+            const float rjp = q[lane * kGramLd + j];
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                if (c > j && c >= lane && c < n) g[c] = __fmaf_rn(-rjp, q[c * kGramLd + j], g[c]);
+        }
+    }
+    __syncthreads();
+    bad = Gr::ballot(bad) != 0ull;
+    dmax = Gr::maxf(dmax);
+    dmin = Gr::minf(dmin);
+    if (bad || !(dmin > kGramGuard * dmax)) return kFallback;
+    return 0;
+}
+
 template <int G, int NMAX>
 __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchLayout& L, const WaveScratch& s,
                                               int m, int n)
@@ -1053,6 +1135,12 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
             if (dmx <= 1e-10 * xmx) return 0;
         }
         return kFallback;
+    } else if (G == kWave && L.gram) {
+        /* R from the Gram matrix; x0 = 0, so the first step is the
+         * semi-normal equations' solution */
+        if (NMAX > 0 || gram_chol(L, s, m, n) == kFallback) return kFallback;
+        for (int i = lane; i < n; i += G) s.b[i] = 0.0;
+        __syncthreads();
     } else {
         /* register bins hold register columns only (their aq32 has room for R alone) */
         if (NMAX > 0 || qr_f32<G>(L, s, m, n) == kFallback) return kFallback;
@@ -1067,10 +1155,12 @@ __device__ __forceinline__ int ls_solve_mixed(const BuildArgs& a, const ScratchL
         for (int i = lane; i < n; i += G) s.b[i] = (double)s.bq32[i];
         __syncthreads();
     }
-    for (int it = 0; it < kRefineSteps; ++it) {
+    const bool gram = G == kWave && L.gram;
+    const int ldr = gram ? L.ldq : ld, steps = gram ? kGramSteps : kRefineSteps;
+    for (int it = 0; it < steps; ++it) {
         csc_apply<G>(a, L, s, m, n, s.b, s.b0, s.r, true);  // r = b0 - A x (FP64)
         csc_apply_t<G>(a, L, s, m, n, s.r, s.g, true);      // g = A^T r
-        normal_solve<G>(q, ld, s.g, n);                      // R^T R d = g, R from the FP32 factor
+        normal_solve<G>(q, ldr, s.g, n);                     // R^T R d = g, R from the FP32 factor
         double dmx = 0.0, xmx = 0.0;
         for (int i = lane; i < n; i += G) {
             double x = s.b[i] + s.g[i];

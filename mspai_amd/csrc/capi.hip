@@ -844,6 +844,7 @@ struct Bin {
     int group;
     int wcap;
     ScratchLayout L;
+    int fp64;  // a bin of a mixed build's chain that runs the FP64 kernel (FP64 layout)
 };
 
 /* A size-bin table: per bin the lane-group width, the |J|, |I|, distinct
@@ -858,6 +859,7 @@ struct BinTable {
     int count;
     BinEdge e[4];
     int mixed;
+    int fp64mask;  // bins (bit t for entry t) that a mixed build runs in FP64 inside its chain
 };
 
 /* Candidate tables of the autotuner; table 0 is the default. */
@@ -941,6 +943,14 @@ static const BinTable kTables[] = {
     {"g64x16u-x20i112-x24i128-x384/m1", 4,
      {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 24, 128, 1024, 2048, 0},
       {64, 40, 384, 4096, 8192, 0}}, 1},
+    /* mixed builds: FP32 in the bulk bin and (the Gram matrix on the matrix
+     * cores) in the widest one, the second bin in FP64 on its eight-row
+     * register QR (f) */
+    {"g64x16u-g64x24i128f-g64x384", 3,
+     {{64, 16, 64, 384, 1024, 0}, {64, 24, 128, 1024, 2048, 0}, {64, 40, 384, 4096, 8192, 0}}, 0, 0x2},
+    {"g64x16u-x20i112f-x24i128f-x384", 4,
+     {{64, 16, 64, 384, 1024, 0}, {64, 20, 112, 1024, 2048, 0}, {64, 24, 128, 1024, 2048, 0},
+      {64, 40, 384, 4096, 8192, 0}}, 0, 0x6},
 };
 constexpr int kNumTables = sizeof(kTables) / sizeof(kTables[0]);
 
@@ -958,11 +968,14 @@ static std::vector<Bin> bin_layouts(const mspai_ctx* ctx, int cap, int maxnew, b
         size_t capI = std::min<size_t>(G.capI, e.I);
         size_t capU = adaptive ? std::min<size_t>(G.capU, e.U) : 1;
         size_t capSort = std::min<size_t>(G.capSort, e.sort);
+        /* a mixed chain's FP64 bin: FP64 layout (the mixed build, not its adjoint) */
+        const int fp64 = (needs & kNeedMixed) && !(needs & kNeedBackward) && ((table.fp64mask >> t) & 1);
+        const int bneeds = fp64 ? (needs & ~kNeedMixed) : needs;
         /* ties of scores across the cut are ordered in the generic kernel only */
-        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, needs, false, e.group,
+        ScratchLayout L = make_layout(capJ, capI, capU, capSort, maxnew, 1, bneeds, false, e.group,
                                       (size_t)capJ * std::max(1, ctx->maxcolnnz));
         if (L.per_wave * (64 / e.group) > ctx->lds_block) continue;
-        out.push_back({e.group, e.wcap, L});
+        out.push_back({e.group, e.wcap, L, fp64});
         if (capJ == cap && L.capI == G.capI && L.capSort >= G.capSort && (!adaptive || L.capU >= G.capU)) break;
     }
     return out;
@@ -1035,8 +1048,8 @@ static size_t tie_bytes_needed(const mspai_ctx* ctx, const std::vector<Bin>& bin
         need = std::max(need, (size_t)ctx->num_cu * waves_per_cu(b, fp64_mode) * (64 / b.group) * b.L.tie_per_group);
     if (mixed)
         for (const Bin& b : bins)
-            need = std::max(need, (size_t)ctx->num_cu * waves_per_cu(b, fp64_mode == 2 ? 3 : 1) * (64 / b.group) *
-                                      b.L.tie_per_group);
+            need = std::max(need, (size_t)ctx->num_cu * waves_per_cu(b, b.fp64 ? fp64_mode : (fp64_mode == 2 ? 3 : 1)) *
+                                      (64 / b.group) * b.L.tie_per_group);
     return need;
 }
 
@@ -1098,10 +1111,11 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
             ++skip;
         /* the columns for FP64 go to the generic kernel in one launch when
          * the last mixed build had few of them and no FP64 bin is wider than
-         * the mixed ones: the FP64 bins would mostly launch on empty queues.
-         * Both routes give the same bits. */
-        const bool direct =
-            fp64_mode == 0 && skip == nf && ctx->mixed_fp64_last >= 0 && ctx->mixed_fp64_last <= kDirectFp64;
+         * the mixed ones: the FP64 bins would mostly launch on empty queues
+         * (not after a Gram bin, whose declined columns are wide: the LDS
+         * bins take them faster). Both routes give the same bits. */
+        const bool direct = fp64_mode == 0 && skip == nf && ctx->mixed_fp64_last >= 0 &&
+                            ctx->mixed_fp64_last <= kDirectFp64 && !Lm.gram;
         const int to = direct ? 0 : skip;  // FP64 queue of the mixed chain's overflow
         launch_classify(a, nb, edges, mq, mcounts, fq, counts, n, ctx->d_counter + 40, ctx->d_counter + 16,
                         ctx->stream);
@@ -1115,12 +1129,15 @@ static int run_chain(mspai_ctx* ctx, BuildArgs& a, const std::vector<Bin>& bins,
              * in the next mixed bin or in the FP64 chain */
             a.save_state = fp64_mode != 2;
             a.n_overflow = b + 1 < nb ? mcounts + b + 1 : counts + to;
-            int per_cu = waves_per_cu(bins[b], fp64_mode == 2 ? 3 : 1);
+            /* a bin the table runs in FP64 inside the mixed chain passes its
+             * overflow on like the others and declines nothing */
+            const int mode = bins[b].fp64 ? fp64_mode : (fp64_mode == 2 ? 3 : 1);
+            int per_cu = waves_per_cu(bins[b], mode);
             const ScratchLayout& L = bins[b].L;
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-            if ((rc = begin_launch(ctx, a, nullptr, fp64_mode == 2 ? 3 : 1, b, &bins[b]))) return rc;
-            launch_binned(a, L, grp, fp64_mode == 2 ? 3 : 1, mq + (size_t)b * n, mcounts + b, mheads + b,
-                          ctx->num_cu * per_cu, ctx->stream, bins[b].wcap);
+            if ((rc = begin_launch(ctx, a, nullptr, mode, b, &bins[b]))) return rc;
+            launch_binned(a, L, grp, mode, mq + (size_t)b * n, mcounts + b, mheads + b, ctx->num_cu * per_cu,
+                          ctx->stream, bins[b].wcap);
             HIPCHK(hipGetLastError());
             if ((rc = end_launch(ctx))) return rc;
             if (b == 0) HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
@@ -1600,7 +1617,7 @@ extern "C" int mspai_autotune(mspai_ctx* ctx, const mspai_opts* opts, int repeat
     if (report) std::memset(report, 0, sizeof(*report));
     const int prec = o.precision == 1 ? 1 : 0;
     for (int t = 0; t < kNumTables && t < MSPAI_MAX_TUNE; ++t) {
-        if ((prec == 1 && kTables[t].count == 0) || (prec == 0 && kTables[t].mixed > 0)) {
+        if ((prec == 1 && kTables[t].count == 0) || (prec == 0 && (kTables[t].mixed > 0 || kTables[t].fp64mask))) {
             /* the mixed kernels run in size bins: a bin-less table is all
              * FP64; a limit on the mixed chain changes nothing in FP64 */
             if (report) report->candidate_ms[t] = -1.0;

@@ -2810,7 +2810,11 @@ __device__ __forceinline__ void wave_append(bool flag, int col, int* out, int* c
  * kernels, 3 for the FP64 64-lane register kernel (each measured faster
  * despite the spill; the mixed 8-wide kernel is slower at 5), and 3 for the
  * 16-lane and the 16-wide 32-lane builds, whose bins LDS holds at 10-12
- * waves per CU. */
+ * waves per CU.  kWaves > 0 sets it for the instantiations binned_kernel
+ * picks by layout: 3 for the 16-wide 64-lane register kernels (bulk bins),
+ * 2 for the eight-row register kernels (|I| <= 128) and the FP64 32-wide one
+ * of bins capped at 8 waves per CU, 1 for bins whose LDS holds one wave per
+ * SIMD. */
 template <int G, bool kLds, int kMode, int NMAX, int kWaves = 0>
 __global__ __launch_bounds__(64, kWaves > 0 ? kWaves
                                  : (G == 32 && NMAX == 8) ? (kMode == 0 ? 5 : 4)

@@ -307,6 +307,69 @@ def _cpu_leg(prob, M64, Mmx, sample):
     return cpu, parity
 
 
+def _apply_leg(b, torch, A, M, reps=50):
+    """The preconditioner application M x (and A x, M^T x) on the device,
+    after the one-time CSR copy, timed with events on a stream of the leg's own.
+    ms_per_call: `reps` products issued back to back from Python (host
+    submission included).  ms_device: the same products queued behind a
+    ~5 ms FP64 matrix product, so that all of them are queued before the first
+    runs and the events see device time only.  Algorithmic bytes per product:
+    12 per stored entry (value + index), 12 per output line (line pointer + y)
+    and 8 per entry of x (read once)."""
+    from mspai_amd.krylov import DeviceOperators
+    ops = DeviceOperators(b)
+    rng = np.random.Generator(np.random.PCG64(3))
+    gate = torch.ones(6144, 6144, dtype=torch.float64, device="cuda")
+    out = {}
+    # a stream of its own: on the null stream (handle 0) mspai_apply runs on
+    # the library's stream and waits for the product before it returns
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        _apply_products(ops, torch, A, M, rng, gate, reps, out)
+    torch.cuda.synchronize()
+    return out
+
+
+def _apply_products(ops, torch, A, M, rng, gate, reps, out):
+    for name, which, transpose, X in (("M_x", 1, False, M), ("A_x", 0, False, A), ("Mt_x", 1, True, M)):
+        n_in = X.shape[0] if transpose else X.shape[1]
+        n_out = X.shape[1] if transpose else X.shape[0]
+        xh = rng.standard_normal(n_in)
+        x = torch.tensor(xh, device="cuda")
+        t0 = time.perf_counter()
+        y = ops._apply(which, x, transpose)  # builds the CSR copy on first use
+        torch.cuda.synchronize()
+        first_ms = 1000.0 * (time.perf_counter() - t0)
+        ref = (X.T @ xh) if transpose else (X @ xh)
+        bitwise = bool(np.array_equal(y.cpu().numpy().view(np.int64), ref.view(np.int64)))
+        for _ in range(3):
+            ops._apply(which, x, transpose)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            ops._apply(which, x, transpose)
+        ev1.record()
+        ev1.synchronize()
+        ms_call = ev0.elapsed_time(ev1) / reps
+        gate.mm(gate)  # keeps the device busy while the products are queued
+        ev0.record()
+        for _ in range(reps):
+            ops._apply(which, x, transpose)
+        ev1.record()
+        ev1.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        t0 = time.perf_counter()
+        for _ in range(5):
+            (X.T @ xh) if transpose else (X @ xh)
+        host_ms = 1000.0 * (time.perf_counter() - t0) / 5
+        alg = 12.0 * X.nnz + 12.0 * n_out + 8.0 * n_in
+        out[name] = {"ms_device": ms, "ms_per_call": ms_call, "nnz": int(X.nnz), "algorithmic_bytes": alg,
+                     "GB_per_s": alg / (ms * 1e-3) / 1e9,
+                     "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "first_call_ms_with_csr_copy": first_ms,
+                     "bitwise_equal_scipy": bitwise, "scipy_host_ms_1_thread": host_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,6 +385,8 @@ def main():
     ap.add_argument("--profile", type=int, default=-1, metavar="TABLE",
                     help="profiling runs: only `steps` builds (after `warmup`) of one precision (--precision "
                          "fp64 or mixed) on bin table TABLE, no autotuning, legs or checks")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="profiling runs: one FP64 build, then only the timed products M x, A x, M^T x")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scale for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -375,6 +440,13 @@ def main():
                           "launches": r["launches"]}))
         return
     print(f"[bench] {args.workload} n={n} nnz(A)={prob.A.nnz}", file=sys.stderr, flush=True)
+    if args.apply_only:
+        b = _builder(mspai_amd, prob)
+        b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                use_mean=prob.use_mean, precision=0, binning=2)
+        print(json.dumps({"apply": _apply_leg(b, torch, prob.A, b.download())}))
+        b.close()
+        return
     b = _builder(mspai_amd, prob)
     base = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
                 use_mean=prob.use_mean)
@@ -411,6 +483,9 @@ def main():
                      "cache_hits_per_build": stc["n_cache_hits"],
                      "bitwise_equal": bool(np.array_equal(bc.download().data.view(np.int64), Mp.data.view(np.int64)))}
         bc.close()
+    apply_run = None
+    if not args.no_legs and torch is not None and args.workload == "random":
+        apply_run = _apply_leg(b, torch, prob.A, b.download())  # M of the last build
     b.close()
     probing = None
     if not args.no_legs and args.workload == "random" and args.precision == "both":
@@ -486,6 +561,7 @@ def main():
         "autotune_mixed": tune_mixed,
         "binned_vs_unbinned": unbinned,
         "cache": cache_run,
+        "apply": apply_run,
         "probing3d": probing,
         "bwd_fwd": bwd,
         "block_spai": block,

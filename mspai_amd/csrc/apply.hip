@@ -61,15 +61,38 @@ __global__ void csr_sort_rows_kernel(int nrows, const int* rowptr, int* colind, 
 }
 
 /* y_i = sum over the stored entries of line i (row of a CSR or column of a
- * CSC array) of val * x[index], in storage order. */
-__global__ void line_dot_kernel(int nlines, const int* ptr, const int* ind, const double* val, const double* x,
-                                double* y)
+ * CSC array) of val * x[index], in storage order.
+ *
+ * A block owns kLineBlock consecutive lines, whose entries are one contiguous
+ * range of ind/val.  The range is read in chunks of kStage entries with
+ * consecutive threads on consecutive entries (coalesced), each product
+ * val * x[index] going to LDS; each thread then adds the products of its own
+ * line that fall in the chunk, in storage order.  Chunks are taken in order,
+ * so every line's sum is the same sequence of roundings as a plain loop over
+ * its entries. */
+constexpr int kLineBlock = 256;
+constexpr int kStage = 2048;
+
+__global__ __launch_bounds__(kLineBlock) void line_dot_kernel(int nlines, const int* ptr, const int* ind,
+                                                              const double* val, const double* x, double* y)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nlines) return;
+    __shared__ double prod[kStage];
+    const int i0 = blockIdx.x * kLineBlock;
+    const int i = i0 + (int)threadIdx.x;
+    const int iend = min(i0 + kLineBlock, nlines);
+    const long long first = ptr[i0], last = ptr[iend];
+    const long long b = i < nlines ? ptr[i] : last;
+    const long long e = i < nlines ? ptr[i + 1] : last;
     double sum = 0.0;
-    for (int e = ptr[i]; e < ptr[i + 1]; ++e) sum = __dadd_rn(sum, __dmul_rn(val[e], x[ind[e]]));
-    y[i] = sum;
+    for (long long c0 = first; c0 < last; c0 += kStage) {
+        const long long c1 = min(c0 + kStage, last);
+        for (long long k = c0 + threadIdx.x; k < c1; k += kLineBlock) prod[k - c0] = __dmul_rn(val[k], x[ind[k]]);
+        __syncthreads();
+        const long long hi = min(e, c1);
+        for (long long k = max(b, c0); k < hi; ++k) sum = __dadd_rn(sum, prod[k - c0]);
+        __syncthreads();
+    }
+    if (i < nlines) y[i] = sum;
 }
 
 __global__ void fill_int_kernel(int* p, long long count, int value)
@@ -112,8 +135,8 @@ void launch_line_dot(int nlines, const int* ptr, const int* ind, const double* v
                      hipStream_t stream)
 {
     if (nlines <= 0) return;
-    hipLaunchKernelGGL(line_dot_kernel, dim3((nlines + 255) / 256), dim3(256), 0, stream, nlines, ptr, ind, val, x,
-                       y);
+    hipLaunchKernelGGL(line_dot_kernel, dim3((nlines + kLineBlock - 1) / kLineBlock), dim3(kLineBlock), 0, stream,
+                       nlines, ptr, ind, val, x, y);
 }
 
 }  // namespace mspai

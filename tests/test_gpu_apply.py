@@ -53,6 +53,43 @@ def test_products_match_scipy_bitwise(gpu, make):
     b.close()
 
 
+def _ragged(nrows, ncols, seed):
+    """Lines of every length the staged product meets: empty lines, a whole
+    block of empty lines, a row and a column longer than two LDS chunks (2048
+    entries each), and lines that straddle chunk boundaries."""
+    import scipy.sparse as sp
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A = sp.random(nrows, ncols, density=0.002, random_state=seed, format="lil")
+    A[7, :] = rng.standard_normal(ncols)
+    A[:, 11] = rng.standard_normal((nrows, 1))
+    A[:, 12] = rng.standard_normal((nrows, 1))
+    A[512:1024, :] = 0
+    A[:, 600:900] = 0
+    A[nrows - 1, :] = 0
+    A = sp.csc_matrix(A)
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("shape", [(5000, 5000), (4500, 2600)])
+def test_products_ragged_lines_bitwise(gpu, shape):
+    from mspai_amd.krylov import DeviceOperators
+    A = _ragged(*shape, seed=11)
+    b = gpu.Builder(0)
+    b.upload(A)
+    ops = DeviceOperators(b)
+    rng = np.random.Generator(np.random.PCG64(9))
+    for transpose in (False, True):
+        x = rng.standard_normal(A.shape[0] if transpose else A.shape[1])
+        # negative-zero products keep the host's signs
+        x[3] = -0.0
+        print(f"[apply ragged {shape} T={transpose}] nnz={A.nnz}", flush=True)
+        y = ops._apply(0, torch.tensor(x, device="cuda"), transpose).cpu().numpy()
+        ref = (A.T @ x) if transpose else (A @ x)
+        assert np.array_equal(y.view(np.int64), ref.view(np.int64))
+    b.close()
+
+
 def test_gmres_with_spai_converges_faster(gpu):
     from mspai_amd.krylov import DeviceOperators, gmres
     prob = gen.config2(128)

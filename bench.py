@@ -370,6 +370,30 @@ def _apply_products(ops, torch, A, M, rng, gate, reps, out):
                      "bitwise_equal_scipy": bitwise, "scipy_host_ms_1_thread": host_ms}
 
 
+def _gmres_leg(b, torch, prob, maxiter=3000):
+    """GMRES(30) to a relative residual of 1e-8 on A x = A x* (x* standard
+    normal), without and with the built M on the right (the reference's
+    KSPGMRES with the PCSHELL applying M, src/main.cpp:228-269): iterations,
+    wall time, and the true relative residual checked on the host."""
+    from mspai_amd.krylov import DeviceOperators, gmres
+    ops = DeviceOperators(b)
+    rng = np.random.Generator(np.random.PCG64(5))
+    rhs = prob.A @ rng.standard_normal(prob.n)
+    bt = torch.tensor(rhs, device="cuda")
+    out = {"restart": 30, "rtol": 1e-8, "maxiter": maxiter, "orthogonalization": "classical Gram-Schmidt"}
+    for name, M in (("none", None), ("spai_right", ops.M)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, it, hist = gmres(ops.A, bt, M, side="right", restart=30, rtol=1e-8, maxiter=maxiter)
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0)
+        xh = x.cpu().numpy()
+        out[name] = {"iterations": it, "wall_ms": ms, "ms_per_iteration": ms / max(it, 1),
+                     "true_rel_residual": float(np.linalg.norm(rhs - prob.A @ xh) / np.linalg.norm(rhs)),
+                     "converged": bool(hist[-1] <= 1e-8)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,7 +491,7 @@ def main():
         r0 = _timed(b, dict(base, precision=0, binning=0), max(2, args.steps // 3), 1)
         unbinned = {"fp64_unbinned_ms_per_build": r0["ms_per_step"], "fp64_binned_ms_per_build": r64["ms_per_step"],
                     "binned_speedup": r0["ms_per_step"] / r64["ms_per_step"]}
-    cache_run = None
+    cache_run = gmres_run = None
     if not args.no_legs and args.workload in ("random", "convdiff2d"):
         # the reference's LRU cache option (-cs): same bits, repeated A(I,J) blocks reuse factors
         c2 = gen.config2() if args.workload == "random" else prob
@@ -477,6 +501,9 @@ def main():
         Mp = bc.download()
         cached = _timed(bc, dict(o2, cache=65536), args.steps, args.warmup)
         stc = bc.build(**dict(o2, cache=65536))
+        if torch is not None:
+            gmres_run = _gmres_leg(bc, torch, c2)
+            gmres_run["workload"] = WORKLOADS["convdiff2d"][1].format(nx=256, n=c2.n)
         cache_run = {"workload": WORKLOADS["convdiff2d"][1].format(nx=256, n=c2.n),
                      "columns_per_s_without": c2.n / (plain["ms_per_step"] * 1e-3),
                      "columns_per_s_with_cache": c2.n / (cached["ms_per_step"] * 1e-3),
@@ -562,6 +589,7 @@ def main():
         "binned_vs_unbinned": unbinned,
         "cache": cache_run,
         "apply": apply_run,
+        "gmres": gmres_run,
         "probing3d": probing,
         "bwd_fwd": bwd,
         "block_spai": block,

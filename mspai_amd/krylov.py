@@ -40,9 +40,35 @@ class DeviceOperators:
         return self._apply(1, x)
 
 
-def gmres(A, b, M=None, side="right", restart=30, rtol=1e-8, maxiter=1000):
-    """Solve A x = b; A and M are callables on device vectors.  Returns
-    (x, iterations, history of relative residual norms as GMRES sees them)."""
+def gmres(A, b, M=None, side="right", restart=30, rtol=1e-8, maxiter=1000, orthog="cgs"):
+    """Solve A x = b; A and M are callables on vectors of b's device.  Returns
+    (x, iterations, history of relative residual norms as GMRES sees them).
+
+    orthog: "cgs" -- classical Gram-Schmidt without refinement, PETSc's
+    default for KSPGMRES (KSPGMRESClassicalGramSchmidtOrthogonalization,
+    refinement "never"): the projections on all basis vectors come from one
+    product with the basis, and each iteration reads the device once; "mgs"
+    -- modified Gram-Schmidt, one projection at a time.
+
+    On a GPU whose current stream is the null stream the solve runs on a
+    stream of its own (the caller's stream waits for it), so that the
+    library's products are queued without a wait each (mspai_apply on the
+    null stream waits for its product before it returns)."""
+    if orthog not in ("cgs", "mgs"):
+        raise ValueError("orthog must be 'cgs' or 'mgs'")
+    if not b.is_cuda or torch.cuda.current_stream(b.device).cuda_stream != 0:
+        return _gmres(A, b, M, side, restart, rtol, maxiter, orthog)
+    cur = torch.cuda.current_stream(b.device)
+    side_stream = torch.cuda.Stream(device=b.device)
+    side_stream.wait_stream(cur)
+    with torch.cuda.stream(side_stream):
+        x, it, hist = _gmres(A, b, M, side, restart, rtol, maxiter, orthog)
+    cur.wait_stream(side_stream)
+    x.record_stream(cur)
+    return x, it, hist
+
+
+def _gmres(A, b, M, side, restart, rtol, maxiter, orthog):
     n = b.numel()
     x = torch.zeros_like(b)
     if M is not None and side == "left":
@@ -72,11 +98,20 @@ def gmres(A, b, M=None, side="right", restart=30, rtol=1e-8, maxiter=1000):
         k_used = 0
         for k in range(m):
             w = op(V[k])
-            for j in range(k + 1):  # modified Gram-Schmidt
-                h = torch.dot(w, V[j]).item()
-                H[j][k] = h
-                w = w - h * V[j]
-            hn = torch.linalg.norm(w).item()
+            if orthog == "cgs":
+                Vk = V[:k + 1]
+                h = torch.mv(Vk, w)
+                w = w - torch.mv(Vk.t(), h)
+                hh = torch.cat((h, torch.linalg.norm(w).view(1))).tolist()  # the one read per iteration
+                for j in range(k + 1):
+                    H[j][k] = hh[j]
+                hn = hh[k + 1]
+            else:
+                for j in range(k + 1):
+                    h = torch.dot(w, V[j]).item()
+                    H[j][k] = h
+                    w = w - h * V[j]
+                hn = torch.linalg.norm(w).item()
             H[k + 1][k] = hn
             if hn != 0.0:
                 V[k + 1] = w / hn
@@ -98,9 +133,8 @@ def gmres(A, b, M=None, side="right", restart=30, rtol=1e-8, maxiter=1000):
         y = [0.0] * k_used  # back substitution H y = g
         for i in range(k_used - 1, -1, -1):
             y[i] = (g[i] - sum(H[i][j] * y[j] for j in range(i + 1, k_used))) / H[i][i]
-        dx = torch.zeros_like(b)
-        for i in range(k_used):
-            dx = dx + y[i] * V[i]
+        dx = torch.mv(V[:k_used].t(), torch.tensor(y, dtype=b.dtype, device=b.device)) if k_used else \
+            torch.zeros_like(b)
         x = x + (M(dx) if (M is not None and side == "right") else dx)
         if hist[-1] <= rtol:
             r = b - A(x)

@@ -394,6 +394,48 @@ def _gmres_leg(b, torch, prob, maxiter=3000):
     return out
 
 
+def _setup_loop_leg(mspai_amd, gen, torch, restart=5, loops=20, its=5):
+    """The loop of the reference's driver (src/main.cpp:256-269) on the
+    256x256 convection-diffusion system: `loops` solves, each a
+    preconditioner setup -- the full adaptive build when count % restart == 0,
+    else the static re-solve on the pattern of the last full build
+    (src/imspai.cpp:663-677, 850-854) -- then GMRES from x = 0 limited to
+    `its` iterations at PETSc's default rtol 1e-5 (KSPSetTolerances(..., 5)).
+    Wall times per setup kind and per solve (the first product with each new
+    M includes its CSR copy)."""
+    from mspai_amd.krylov import DeviceOperators, gmres
+    prob = gen.config2()
+    b = _builder(mspai_amd, prob)
+    ops = DeviceOperators(b)
+    bt = torch.tensor(prob.A @ np.ones(prob.n), device="cuda")
+    full, resolve, solve, res = [], [], [], []
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for count in range(loops):
+        t0 = time.perf_counter()
+        if count % restart == 0:
+            b.set_pattern(prob.P)
+            b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
+                    precision=0, binning=1)
+            full.append(1000.0 * (time.perf_counter() - t0))
+        else:
+            b.reuse_pattern()
+            b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=0, precision=0, binning=1)
+            resolve.append(1000.0 * (time.perf_counter() - t0))
+        t1 = time.perf_counter()
+        _, it, hist = gmres(ops.A, bt, ops.M, side="right", restart=30, rtol=1e-5, maxiter=its)
+        torch.cuda.synchronize()
+        solve.append(1000.0 * (time.perf_counter() - t1))
+        res.append(hist[-1])
+    total = 1000.0 * (time.perf_counter() - t_all)
+    nnz_m = int(b.download().nnz)
+    b.close()
+    return {"workload": WORKLOADS["convdiff2d"][1].format(nx=256, n=prob.n), "loops": loops, "restart": restart,
+            "gmres_iterations_per_solve": its, "total_ms": total,
+            "full_build_ms_mean": float(np.mean(full)), "resolve_ms_mean": float(np.mean(resolve)),
+            "solve_ms_mean": float(np.mean(solve)), "last_rel_residual": float(res[-1]), "nnz_M": nnz_m}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -510,9 +552,10 @@ def main():
                      "cache_hits_per_build": stc["n_cache_hits"],
                      "bitwise_equal": bool(np.array_equal(bc.download().data.view(np.int64), Mp.data.view(np.int64)))}
         bc.close()
-    apply_run = None
+    apply_run = setup_loop = None
     if not args.no_legs and torch is not None and args.workload == "random":
         apply_run = _apply_leg(b, torch, prob.A, b.download())  # M of the last build
+        setup_loop = _setup_loop_leg(mspai_amd, gen, torch)
     b.close()
     probing = None
     if not args.no_legs and args.workload == "random" and args.precision == "both":
@@ -590,6 +633,7 @@ def main():
         "cache": cache_run,
         "apply": apply_run,
         "gmres": gmres_run,
+        "setup_solve_loop": setup_loop,
         "probing3d": probing,
         "bwd_fwd": bwd,
         "block_spai": block,

@@ -6,9 +6,10 @@
  *
  * A and M are held in CSC.  y = op^T x reads the columns directly (one
  * sequential dot product per column).  y = op x uses a CSR copy built on the
- * device on first use: row counts, a scan, a fill, then every row sorted by
- * column index, so that each dot product sums its terms in increasing
- * column order -- the order of a column-by-column CSC product on the host.
+ * device on first use: a stable radix sort of the entries by row -- taken
+ * in CSC order, so column order within a row -- gives each row's entries in
+ * increasing column order, the order of a column-by-column CSC product on
+ * the host, and the sorted rows give the row pointers.
  * Terms are multiplied and added separately (no fused multiply-add).
  */
 #include <hip/hip_runtime.h>
@@ -18,46 +19,40 @@
 namespace mspai {
 namespace {
 
-/* counts[r] += 1 for every stored entry of row r (counts zeroed). */
-__global__ void row_counts_kernel(const int* rowind, long long nnz, int* counts)
-{
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nnz) atomicAdd(counts + rowind[e], 1);
-}
-
-/* Scatter the entries of column j into their rows (any order within a row;
- * cursor zeroed). */
-__global__ void csr_fill_kernel(const int* colptr, const int* rowind, const double* val, int ncols,
-                                const int* rowptr, int* cursor, int* colind, double* cval)
+/* Entry e of column j: col_of[e] = j, order[e] = e (the payload of the
+ * stable sort of the entries by row). */
+__global__ void csr_entry_cols_kernel(const int* colptr, int ncols, int* col_of, int* order)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ncols) return;
     for (int e = colptr[j]; e < colptr[j + 1]; ++e) {
-        const int r = rowind[e];
-        const int at = rowptr[r] + atomicAdd(cursor + r, 1);
-        colind[at] = j;
-        cval[at] = val[e];
+        col_of[e] = j;
+        order[e] = e;
     }
 }
 
-/* Insertion sort of every row by column index (rows are short). */
-__global__ void csr_sort_rows_kernel(int nrows, const int* rowptr, int* colind, double* cval)
+/* Row pointers from the rows of the entries sorted by row: entry k (and the
+ * end, k = nnz) starts every row after the previous entry's row up to its
+ * own, so each of rowptr[0..nrows] is written once; empty rows get the
+ * start of the next stored row. */
+__global__ void csr_row_starts_kernel(long long nnz, int nrows, const unsigned* rows_sorted, int* rowptr)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
-    const int b = rowptr[r], e = rowptr[r + 1];
-    for (int i = b + 1; i < e; ++i) {
-        const int c = colind[i];
-        const double v = cval[i];
-        int k = i - 1;
-        while (k >= b && colind[k] > c) {
-            colind[k + 1] = colind[k];
-            cval[k + 1] = cval[k];
-            --k;
-        }
-        colind[k + 1] = c;
-        cval[k + 1] = v;
-    }
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    const int prev = k == 0 ? -1 : (int)rows_sorted[k - 1];
+    const int cur = k == nnz ? nrows : (int)rows_sorted[k];
+    for (int r = prev + 1; r <= cur; ++r) rowptr[r] = (int)k;
+}
+
+/* CSR entry k is CSC entry perm[k]: its column and its value. */
+__global__ void csr_gather_kernel(long long nnz, const int* perm, const int* col_of, const double* val,
+                                  int* colind, double* cval)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int e = perm[k];
+    colind[k] = col_of[e];
+    cval[k] = val[e];
 }
 
 /* y_i = sum over the stored entries of line i (row of a CSR or column of a
@@ -109,26 +104,25 @@ void launch_fill_int(int* p, long long count, int value, hipStream_t stream)
     hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, p, count, value);
 }
 
-void launch_row_counts(const int* rowind, long long nnz, int* counts, hipStream_t stream)
-{
-    if (nnz <= 0) return;
-    hipLaunchKernelGGL(row_counts_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, stream, rowind, nnz,
-                       counts);
-}
-
-void launch_csr_fill(const int* colptr, const int* rowind, const double* val, int ncols, const int* rowptr,
-                     int* cursor, int* colind, double* cval, hipStream_t stream)
+void launch_csr_entry_cols(const int* colptr, int ncols, int* col_of, int* order, hipStream_t stream)
 {
     if (ncols <= 0) return;
-    hipLaunchKernelGGL(csr_fill_kernel, dim3((ncols + 255) / 256), dim3(256), 0, stream, colptr, rowind, val, ncols,
-                       rowptr, cursor, colind, cval);
+    hipLaunchKernelGGL(csr_entry_cols_kernel, dim3((ncols + 255) / 256), dim3(256), 0, stream, colptr, ncols, col_of,
+                       order);
 }
 
-void launch_csr_sort_rows(int nrows, const int* rowptr, int* colind, double* cval, hipStream_t stream)
+void launch_csr_row_starts(long long nnz, int nrows, const unsigned* rows_sorted, int* rowptr, hipStream_t stream)
 {
-    if (nrows <= 0) return;
-    hipLaunchKernelGGL(csr_sort_rows_kernel, dim3((nrows + 255) / 256), dim3(256), 0, stream, nrows, rowptr, colind,
-                       cval);
+    hipLaunchKernelGGL(csr_row_starts_kernel, dim3((unsigned)((nnz + 1 + 255) / 256)), dim3(256), 0, stream, nnz,
+                       nrows, rows_sorted, rowptr);
+}
+
+void launch_csr_gather(long long nnz, const int* perm, const int* col_of, const double* val, int* colind,
+                       double* cval, hipStream_t stream)
+{
+    if (nnz <= 0) return;
+    hipLaunchKernelGGL(csr_gather_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, stream, nnz, perm,
+                       col_of, val, colind, cval);
 }
 
 void launch_line_dot(int nlines, const int* ptr, const int* ind, const double* val, const double* x, double* y,

@@ -174,10 +174,14 @@ struct mspai_ctx {
     // CSR copies for mspai_apply (built on first use; valid while *_csr_ok)
     int *d_acsr_ptr = nullptr, *d_acsr_ind = nullptr, *d_mcsr_ptr = nullptr, *d_mcsr_ind = nullptr;
     double *d_acsr_val = nullptr, *d_mcsr_val = nullptr;
-    int* d_csr_cursor = nullptr;
+    int acsr_rows_cap = 0, mcsr_rows_cap = 0;
+    long long acsr_nnz_cap = 0, mcsr_nnz_cap = 0;
+    int* d_csr_tmp = nullptr;  // 4 x csr_tmp_cap ints: columns, order, sorted rows, permutation
+    long long csr_tmp_cap = 0;
     bool a_csr_ok = false, m_csr_ok = false;
     bool ajn_ok = false;  // d_ajn holds the column norms of the current values
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // [4..5]: first size bin
+    hipEvent_t csr_ev = nullptr;  // orders a CSR copy on the context's stream with the caller's stream
     /* per-launch profile of the last build: events around every column-kernel
      * launch, and (builds with counters) the totals of the columns each finished */
     hipEvent_t lev[2 * kMaxLaunch] = {};
@@ -325,6 +329,10 @@ extern "C" int mspai_ctx_create(int device, mspai_ctx** out)
             delete ctx;
             return MSPAI_ERR_HIP;
         }
+    if (hipEventCreateWithFlags(&ctx->csr_ev, hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return MSPAI_ERR_HIP;
+    }
     if (device < kMaxDevices) {
         std::lock_guard<std::mutex> lock(g_noalloc_mu);
         if (!g_noalloc_ptr[device]) {
@@ -395,12 +403,13 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_mcsr_ptr);
     dfree(ctx->d_mcsr_ind);
     dfree(ctx->d_mcsr_val);
-    dfree(ctx->d_csr_cursor);
+    dfree(ctx->d_csr_tmp);
     block_free(ctx);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->lev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->csr_ev) (void)hipEventDestroy(ctx->csr_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MSPAI_OK;
@@ -1749,35 +1758,66 @@ extern "C" int mspai_download_column_info(mspai_ctx* ctx, double* res_norm, int*
     return MSPAI_OK;
 }
 
-/* CSR copy of a CSC matrix (nrows x ncols) on the stream: row counts, scan,
- * fill, rows sorted by column. */
-static int build_csr(mspai_ctx* ctx, int nrows, int ncols, long long nnz, const int* colptr, const int* rowind,
-                     const double* val, int*& rp, int*& ri, double*& rv, hipStream_t st)
+/* Device buffer of at least `need` elements, grown (never shrunk) in steps
+ * of a quarter so that copies of similar size reuse it. */
+template <typename T>
+static int grow_buffer(mspai_ctx* ctx, T*& p, long long need, long long& cap)
 {
-    dfree(rp);
-    dfree(ri);
-    dfree(rv);
-    dfree(ctx->d_csr_cursor);
-    HIPCHK(hipMalloc((void**)&rp, (nrows + 1) * sizeof(int)));
-    HIPCHK(hipMalloc((void**)&ri, std::max<long long>(nnz, 1) * sizeof(int)));
-    HIPCHK(hipMalloc((void**)&rv, std::max<long long>(nnz, 1) * sizeof(double)));
-    HIPCHK(hipMalloc((void**)&ctx->d_csr_cursor, std::max(nrows, 1) * sizeof(int)));
-    size_t need = 0;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, need, ctx->d_csr_cursor, rp + 1, nrows, st));
-    if (need > ctx->cub_bytes) {
-        if (ctx->d_cub) (void)hipFree(ctx->d_cub);
-        ctx->d_cub = nullptr;
-        HIPCHK(hipMalloc(&ctx->d_cub, need));
-        ctx->cub_bytes = need;
+    if (need <= cap && p) return MSPAI_OK;
+    dfree(p);
+    cap = 0;
+    const long long want = std::max<long long>(need + need / 4, 1);
+    HIPCHK(hipMalloc((void**)&p, want * sizeof(T)));
+    cap = want;
+    return MSPAI_OK;
+}
+
+/* CSR copy of a CSC matrix (nrows x ncols) on the stream: a stable radix
+ * sort of the entries by row, taken in CSC order (columns ascending), leaves
+ * each row's entries in increasing column order; the sorted rows give the
+ * row pointers and a gather moves columns and values. */
+static int build_csr(mspai_ctx* ctx, int nrows, int ncols, long long nnz, const int* colptr, const int* rowind,
+                     const double* val, int*& rp, int*& ri, double*& rv, int& rows_cap, long long& nnz_cap,
+                     hipStream_t st)
+{
+    int rc;
+    long long rcap = rows_cap;
+    if ((rc = grow_buffer(ctx, rp, (long long)nrows + 1, rcap))) return rc;
+    rows_cap = (int)std::min<long long>(rcap, 0x7fffffff);
+    long long ncap = nnz_cap;
+    if (!ri || !rv || nnz > nnz_cap) {
+        dfree(ri);
+        dfree(rv);
+        long long c1 = 0, c2 = 0;
+        if ((rc = grow_buffer(ctx, ri, nnz, c1)) || (rc = grow_buffer(ctx, rv, nnz, c2))) return rc;
+        ncap = std::min(c1, c2);
     }
-    launch_fill_int(ctx->d_csr_cursor, nrows, 0, st);
-    launch_fill_int(rp, 1, 0, st);
-    launch_row_counts(rowind, nnz, ctx->d_csr_cursor, st);
-    size_t bytes = ctx->cub_bytes;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(ctx->d_cub, bytes, ctx->d_csr_cursor, rp + 1, nrows, st));
-    launch_fill_int(ctx->d_csr_cursor, nrows, 0, st);
-    launch_csr_fill(colptr, rowind, val, ncols, rp, ctx->d_csr_cursor, ri, rv, st);
-    launch_csr_sort_rows(nrows, rp, ri, rv, st);
+    nnz_cap = ncap;
+    if ((rc = grow_buffer(ctx, ctx->d_csr_tmp, 4 * nnz, ctx->csr_tmp_cap))) return rc;
+    const long long tcap = ctx->csr_tmp_cap / 4;
+    int* const col_of = ctx->d_csr_tmp;
+    int* const order = col_of + tcap;
+    unsigned* const rows_sorted = reinterpret_cast<unsigned*>(order + tcap);
+    int* const perm = reinterpret_cast<int*>(rows_sorted + tcap);
+    const int bits = 32 - __builtin_clz((unsigned)std::max(nrows - 1, 1));
+    if (nnz > 0) {
+        size_t need = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, reinterpret_cast<const unsigned*>(rowind),
+                                                  rows_sorted, order, perm, (int)nnz, 0, bits, st));
+        if (need > ctx->cub_bytes) {
+            if (ctx->d_cub) (void)hipFree(ctx->d_cub);
+            ctx->d_cub = nullptr;
+            ctx->cub_bytes = 0;
+            HIPCHK(hipMalloc(&ctx->d_cub, need));
+            ctx->cub_bytes = need;
+        }
+        launch_csr_entry_cols(colptr, ncols, col_of, order, st);
+        size_t bytes = ctx->cub_bytes;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->d_cub, bytes, reinterpret_cast<const unsigned*>(rowind),
+                                                  rows_sorted, order, perm, (int)nnz, 0, bits, st));
+        launch_csr_gather(nnz, perm, col_of, val, ri, rv, st);
+    }
+    launch_csr_row_starts(nnz, nrows, rows_sorted, rp, st);
     HIPCHK(hipGetLastError());
     return MSPAI_OK;
 }
@@ -1804,7 +1844,20 @@ extern "C" int mspai_apply(mspai_ctx* ctx, int which, int transpose, const doubl
         double*& rv = which == 0 ? ctx->d_acsr_val : ctx->d_mcsr_val;
         if (!ok) {
             const long long nnz = which == 0 ? ctx->nnz : ctx->nnzM;
-            if ((rc = build_csr(ctx, nrows, n, nnz, cp, ri, cv, rp, rind, rv, st))) return rc;
+            int& rcap = which == 0 ? ctx->acsr_rows_cap : ctx->mcsr_rows_cap;
+            long long& ncap = which == 0 ? ctx->acsr_nnz_cap : ctx->mcsr_nnz_cap;
+            /* the copy runs on the context's stream, after the caller's
+             * work on `st` and before the product: its scan and sort share
+             * the context's library workspace with the builds */
+            if (st != ctx->stream) {
+                HIPCHK(hipEventRecord(ctx->csr_ev, st));
+                HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->csr_ev, 0));
+            }
+            if ((rc = build_csr(ctx, nrows, n, nnz, cp, ri, cv, rp, rind, rv, rcap, ncap, ctx->stream))) return rc;
+            if (st != ctx->stream) {
+                HIPCHK(hipEventRecord(ctx->csr_ev, ctx->stream));
+                HIPCHK(hipStreamWaitEvent(st, ctx->csr_ev, 0));
+            }
             ok = true;
         }
         launch_line_dot(nrows, rp, rind, rv, d_x, d_y, st);

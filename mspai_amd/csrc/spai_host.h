@@ -40,12 +40,13 @@ void launch_fp64_generic(const BuildArgs& a, const ScratchLayout& L, int mode, c
 
 /* Sparse products (apply.hip). */
 void launch_fill_int(int* p, long long count, int value, hipStream_t stream);
-/* counts[r] += number of entries in row r (counts zeroed beforehand). */
-void launch_row_counts(const int* rowind, long long nnz, int* counts, hipStream_t stream);
-/* CSC -> CSR entries in rows given rowptr (cursor zeroed); rows unsorted. */
-void launch_csr_fill(const int* colptr, const int* rowind, const double* val, int ncols, const int* rowptr,
-                     int* cursor, int* colind, double* cval, hipStream_t stream);
-void launch_csr_sort_rows(int nrows, const int* rowptr, int* colind, double* cval, hipStream_t stream);
+/* col_of[e] = column of CSC entry e, order[e] = e. */
+void launch_csr_entry_cols(const int* colptr, int ncols, int* col_of, int* order, hipStream_t stream);
+/* rowptr[0..nrows] from the row indices of the entries sorted by row. */
+void launch_csr_row_starts(long long nnz, int nrows, const unsigned* rows_sorted, int* rowptr, hipStream_t stream);
+/* colind[k] = col_of[perm[k]], cval[k] = val[perm[k]]. */
+void launch_csr_gather(long long nnz, const int* perm, const int* col_of, const double* val, int* colind,
+                       double* cval, hipStream_t stream);
 /* y[i] = sum over line i of val * x[ind] in storage order (mul, then add). */
 void launch_line_dot(int nlines, const int* ptr, const int* ind, const double* val, const double* x, double* y,
                      hipStream_t stream);

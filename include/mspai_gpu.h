@@ -250,7 +250,11 @@ int mspai_bin_counts(const mspai_ctx* ctx, int* nbins, int* routed, int* queued)
  * it.  Every y_i sums its terms in increasing index order with separate
  * multiply and add.  `stream` (a hipStream_t, or NULL) orders the product
  * after the caller's work on that stream; with NULL the call runs on the
- * context's stream and returns when y is written. */
+ * context's stream and returns when y is written.  The first plain product
+ * after A or M changed builds a CSR copy on the context's stream, ordered
+ * after the caller's stream by an event.  Calls that change A or M
+ * (upload, update_values, a build) must not overlap a product still in
+ * flight on another stream: synchronize that stream first. */
 int mspai_apply(mspai_ctx* ctx, int which, int transpose, const double* d_x, double* d_y, void* stream);
 
 

@@ -513,6 +513,7 @@ def main():
         print(json.dumps({"apply": _apply_leg(b, torch, prob.A, b.download())}))
         b.close()
         return
+    free0 = torch.cuda.mem_get_info()[0] if torch is not None else None
     b = _builder(mspai_amd, prob)
     base = dict(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps,
                 use_mean=prob.use_mean)
@@ -527,6 +528,10 @@ def main():
         rmx = _timed(b, dict(base, precision=1, binning=2), args.steps, args.warmup)
         cmx = _counted(b, dict(base, precision=1, binning=2))
         print(f"[bench] mixed {rmx['ms_per_step']:.3f} ms/build", file=sys.stderr, flush=True)
+    # device memory the builder holds after autotuning and the timed builds
+    # (A, patterns, scratch, output, workspaces): its buffers only grow, so
+    # this is its peak over every bin table tried
+    hbm_held = (free0 - torch.cuda.mem_get_info()[0]) if free0 is not None else None
     unbinned = None
     if not args.no_legs and args.workload == "random":
         # the generic kernel alone (no size bins): the unbinned baseline
@@ -617,6 +622,7 @@ def main():
         "mixed_over_fp64": (r64["ms_per_step"] / rmx["ms_per_step"]) if rmx is not None else None,
         "mixed_check": mixed_check,
         "fro_AM_minus_I": fro_gpu,
+        "device_memory_held_bytes": hbm_held,
         "near_margin_decisions_fp64": {"score_at_mean": int(near[0]), "maxnew_cut": int(near[1]),
                                        "residual_at_eps": int(near[2]),
                                        "margin": "at most 1e-12 relative"},

@@ -101,6 +101,13 @@ int mspai_update_values(mspai_ctx* ctx, const double* val);
  * the diagonal pattern (-pp 1, src/Pattern.cpp:378-423). */
 int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* rowind);
 
+/* Upper pattern UP (-up, ncols columns, CSC pattern with sorted distinct
+ * indices below ncols): each augmentation step of column k keeps only the
+ * candidates in UP(:,k) (J~ = J~ n UP(:,k), Index_Set::Set_Intersection in
+ * Augment_Sparsity, src/Spai_Sub.imp:1072-1094).  NULL pointers remove it.
+ * Scalar SPAI only; a re-upload with another shape drops it. */
+int mspai_set_upper_pattern(mspai_ctx* ctx, const int* colptr, const int* rowind);
+
 /* Start pattern = the pattern of the last M built in this context: the
  * P_Memory reuse of repeated setups (P_Memory = M_REAL->To_Pattern,
  * src/imspai.cpp:850-854; between pattern recomputations the reference

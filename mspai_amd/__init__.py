@@ -20,7 +20,8 @@ NWORK = 8
 COL_SINGULAR, COL_UNDERDET, COL_NAN_RHO, COL_FALLBACK, COL_K_NOT_IN_I = 1, 2, 4, 8, 16
 
 EXPORTED = ["mspai_default_opts", "mspai_ctx_create", "mspai_ctx_destroy", "mspai_last_error",
-            "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_reuse_pattern",
+            "mspai_upload_csc", "mspai_update_values", "mspai_set_pattern", "mspai_set_upper_pattern",
+            "mspai_reuse_pattern",
             "mspai_set_target",
             "mspai_build", "mspai_build_noalloc", "mspai_result_nnz", "mspai_download_csc",
             "mspai_download_column_info", "mspai_work_totals", "mspai_bin_counts", "mspai_launch_stats",
@@ -139,6 +140,20 @@ class Builder:
         P.sort_indices()
         self._chk(self.lib.mspai_set_pattern(self.h, _p(_i32(P.indptr)), _p(_i32(P.indices))),
                   "mspai_set_pattern")
+
+    def set_upper_pattern(self, UP=None):
+        """Upper pattern (-up): column k's candidates are kept only where
+        UP(:, k) has a stored entry.  None removes it."""
+        if UP is None:
+            self._chk(self.lib.mspai_set_upper_pattern(self.h, None, None), "mspai_set_upper_pattern")
+            return
+        U = sp.csc_matrix(UP)
+        U.sort_indices()
+        U.sum_duplicates()
+        if U.shape[1] != self.shape[1]:
+            raise ValueError("the upper pattern needs the matrix's column count")
+        self._chk(self.lib.mspai_set_upper_pattern(self.h, _p(_i32(U.indptr)), _p(_i32(U.indices))),
+                  "mspai_set_upper_pattern")
 
     def reuse_pattern(self):
         """Start pattern = the last built M's pattern (the reference's P_Memory

@@ -141,6 +141,8 @@ struct mspai_ctx {
     int *d_p_ptr = nullptr, *d_p_ind = nullptr;
     int plen_max = 0;
     int p_cols = 0;             // columns the start pattern was set for
+    int *d_up_ptr = nullptr, *d_up_ind = nullptr;  // upper pattern (-up), CSC; nullptr: none
+    int up_cols = 0;
     bool pattern_set = false;   // set by the caller (the implicit diagonal of a build is not)
     // target
     int *d_b_ptr = nullptr, *d_b_ind = nullptr;
@@ -370,6 +372,8 @@ extern "C" int mspai_ctx_destroy(mspai_ctx* ctx)
     dfree(ctx->d_rp_val);
     dfree(ctx->d_p_ptr);
     dfree(ctx->d_p_ind);
+    dfree(ctx->d_up_ptr);
+    dfree(ctx->d_up_ind);
     dfree(ctx->d_b_ptr);
     dfree(ctx->d_b_ind);
     dfree(ctx->d_b_val);
@@ -514,6 +518,9 @@ extern "C" int mspai_upload_csc(mspai_ctx* ctx, int nrows, int ncols, const int*
         ctx->p_cols = 0;
     }
     if (reshaped) {
+        dfree(ctx->d_up_ptr);
+        dfree(ctx->d_up_ind);
+        ctx->up_cols = 0;
         dfree(ctx->d_b_ptr);
         dfree(ctx->d_b_ind);
         dfree(ctx->d_b_val);
@@ -533,6 +540,10 @@ static int check_shapes(mspai_ctx* ctx)
                                               " columns, the matrix has " + std::to_string(pc));
     if (ctx->d_b_ptr && (ctx->b_rows != ctx->nrows || ctx->b_cols != ctx->ncols))
         return fail(ctx, MSPAI_ERR_STATE, "target was set for another matrix shape");
+    if (ctx->d_up_ptr && ctx->up_cols != ctx->ncols)
+        return fail(ctx, MSPAI_ERR_STATE, "upper pattern was set for another matrix shape");
+    if (ctx->d_up_ptr && ctx->block_bs != 1)
+        return fail(ctx, MSPAI_ERR_ARG, "the upper pattern applies to scalar SPAI only");
     return MSPAI_OK;
 }
 
@@ -582,6 +593,40 @@ extern "C" int mspai_set_pattern(mspai_ctx* ctx, const int* colptr, const int* r
     ctx->plen_max = pmax;
     ctx->p_cols = n;
     ctx->pattern_set = true;
+    return MSPAI_OK;
+}
+
+/* -up: the candidates of column k are restricted to UP(:,k)
+ * (src/Spai_Sub.imp:1072-1094). */
+extern "C" int mspai_set_upper_pattern(mspai_ctx* ctx, const int* colptr, const int* rowind)
+{
+    if (!ctx) return MSPAI_ERR_ARG;
+    if (ctx->ncols <= 0) return fail(ctx, MSPAI_ERR_STATE, "set_upper_pattern: upload the matrix first");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!colptr || !rowind) {
+        dfree(ctx->d_up_ptr);
+        dfree(ctx->d_up_ind);
+        ctx->up_cols = 0;
+        return MSPAI_OK;
+    }
+    if (ctx->block_bs != 1) return fail(ctx, MSPAI_ERR_ARG, "set_upper_pattern: scalar SPAI only");
+    const int n = ctx->ncols;
+    if (colptr[0] != 0) return fail(ctx, MSPAI_ERR_ARG, "set_upper_pattern: colptr must start at 0");
+    for (int j = 0; j < n; ++j) {
+        const int a = colptr[j], b = colptr[j + 1];
+        if (b < a) return fail(ctx, MSPAI_ERR_ARG, "set_upper_pattern: colptr not monotone");
+        for (int e = a; e < b; ++e) {
+            if (rowind[e] < 0 || rowind[e] >= n)
+                return fail(ctx, MSPAI_ERR_ARG, "set_upper_pattern: index out of range");
+            if (e > a && rowind[e] <= rowind[e - 1])
+                return fail(ctx, MSPAI_ERR_ARG, "set_upper_pattern: indices must be sorted and distinct");
+        }
+    }
+    int rc;
+    if ((rc = upload(ctx, ctx->d_up_ptr, colptr, n + 1))) return rc;
+    if ((rc = upload(ctx, ctx->d_up_ind, rowind, (size_t)colptr[n]))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->up_cols = n;
     return MSPAI_OK;
 }
 
@@ -1311,7 +1356,7 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     const int needs = (qrup ? kNeedQrUpdate : 0) | (opts->precision == 1 && !qrup ? kNeedMixed : 0);
     const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive, needs);
     std::vector<Bin> bins, fbins;
-    if (opts->binning) {
+    if (opts->binning && !ctx->d_up_ptr) {  // an upper pattern runs unbinned
         int t = table_for(ctx, opts);
         bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t], needs);
         /* the FP64 chain of a mixed build: the same table in FP64 layouts
@@ -1336,6 +1381,8 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     }
     a.p_ptr = ctx->d_p_ptr;
     a.p_ind = ctx->d_p_ind;
+    a.up_ptr = ctx->d_up_ptr;
+    a.up_ind = ctx->d_up_ind;
     a.epsilon = opts->epsilon;
     a.maxnew = opts->maxnew;
     a.ns = ns;
@@ -1444,7 +1491,7 @@ extern "C" int mspai_build_noalloc(mspai_ctx* ctx, const mspai_opts* opts, int* 
     const int needs = (qrup ? kNeedQrUpdate : 0) | (opts->precision == 1 && !qrup ? kNeedMixed : 0);
     const ScratchLayout G = generic_layout(ctx, cap, opts->maxnew, adaptive, needs);
     std::vector<Bin> bins, fbins;
-    if (opts->binning) {
+    if (opts->binning && !ctx->d_up_ptr) {  // an upper pattern runs unbinned
         int t = table_for(ctx, opts);
         bins = bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t], needs);
         fbins = (needs & kNeedMixed) ? bin_layouts(ctx, cap, opts->maxnew, adaptive, G, kTables[t],

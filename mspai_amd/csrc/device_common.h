@@ -94,6 +94,8 @@ struct BuildArgs {
     int rp_maxlen;          // longest row of the pattern (groups of fewer lanes score by columns)
     const int* p_ptr;       // start pattern (CSC)
     const int* p_ind;
+    const int* up_ptr;      // upper pattern (CSC, -up): candidates of column k in UP(:,k); nullptr: none
+    const int* up_ind;
     DevCsc B;               // target; B.colptr == nullptr means identity
     const double* ajn;      // ||a_j|| as Euclidean_Norm computes it
     double epsilon;

@@ -1831,6 +1831,31 @@ __device__ __forceinline__ int augment(const BuildArgs& a, const ScratchLayout& 
         }
         __syncthreads();
     }
+    if (kTies && a.up_ptr && c > 0) {
+        /* J~ n UP(:,col) (Set_Intersection, src/Spai_Sub.imp:1072-1094), in
+         * the generic kernel only (builds with an upper pattern run
+         * unbinned; the size-bin kernels keep their register budgets): each
+         * candidate is searched in the column of the upper pattern, the kept
+         * ones compacted in order (a lane's write never passes a slot not yet
+         * read).  The scores then run over the candidates' columns. */
+        const int u0 = a.up_ptr[col], ulen = a.up_ptr[col + 1] - u0;
+        int kept = 0;
+        for (int base = 0; base < c; base += G) {
+            const int t = base + lane;
+            const int j = t < c ? s.jt[t] : 0;
+            bool keep = false;
+            if (t < c && ulen > 0) {
+                const int p = lower_bound_i(a.up_ind + u0, ulen, j);
+                keep = p < ulen && a.up_ind[u0 + p] == j;
+            }
+            const unsigned long long mask = Gr::ballot(keep);
+            if (keep) s.jt[kept + __popcll(mask & Gr::lanes_below())] = j;
+            kept += __popcll(mask);
+        }
+        __syncthreads();
+        c = kept;
+        by_rows = false;
+    }
     if (c == 0) {
         if (wk && lane == 0) wk[3] += acc3;
         return -1;

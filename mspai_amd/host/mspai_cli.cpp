@@ -90,7 +90,8 @@ int main(int argc, char** argv)
         std::fprintf(stderr,
                      "usage: %s <matrix.mtx> [<pattern.mtx|-1|-2> [<target.mtx|-1>]] [-ep eps] [-mn maxnew] "
                      "[-ns steps] [-um 0|1] [-qr 0|2] [-cs entries] [-ch 0|1] [-bs size] [-pp 1|2] [-Ce file] "
-                     "[-Be file] [-rho r] [-schur 0] [-out file] [-vb 0|1] [-prec fp64|mixed] [-bins 0|1|2]\n",
+                     "[-Be file] [-rho r] [-schur 0] [-up file|1|2] [-out file] [-vb 0|1] [-prec fp64|mixed] "
+                     "[-bins 0|1|2]\n",
                      argv[0]);
         return 1;
     }
@@ -98,7 +99,7 @@ int main(int argc, char** argv)
     try {
         mspai_opts o;
         mspai_default_opts(&o);
-        std::string out = "precond.mtx", pattern_file, target_file, ce_file, be_file;
+        std::string out = "precond.mtx", pattern_file, target_file, ce_file, be_file, up_arg;
         int pp = 1, ch = 0, bs = 1, schur = 0;
         double rho = 1.0;
         int k = 2;
@@ -131,6 +132,7 @@ int main(int argc, char** argv)
             else if (key == "-Be") be_file = v;
             else if (key == "-rho") rho = std::atof(v);
             else if (key == "-schur") schur = std::atoi(v);
+            else if (key == "-up") up_arg = v;
             else if (key == "-out") out = v;
             else if (key == "-vb") o.verbose = std::atoi(v);
             else if (key == "-prec") o.precision = std::strcmp(v, "mixed") == 0 ? 1 : 0;
@@ -181,6 +183,30 @@ int main(int argc, char** argv)
         }
         if (targeted)
             check(mspai_set_target(ctx, B.colptr.data(), B.rowind.data(), B.val.data()), ctx, "mspai_set_target");
+        if (!up_arg.empty()) {
+            /* -up: a MatrixMarket pattern file, 1 (diagonal) or 2 (the pattern
+             * of A), Generate_Pattern in src/imspai.cpp:797-810 */
+            if (bs != 1) throw std::runtime_error("-up: scalar SPAI only");
+            std::vector<int> uc(n + 1, 0), ui;
+            if (up_arg == "1") {
+                for (int j = 0; j < n; ++j) {
+                    ui.push_back(j);
+                    uc[j + 1] = j + 1;
+                }
+            } else if (up_arg == "2") {
+                for (int j = 0; j < n; ++j) {
+                    for (int e = A.colptr[j]; e < A.colptr[j + 1] && A.rowind[e] < n; ++e) ui.push_back(A.rowind[e]);
+                    uc[j + 1] = (int)ui.size();
+                }
+            } else {
+                mspai::CscMatrix U = mspai::read_matrix_market(up_arg);
+                if (U.ncols != n) throw std::runtime_error("-up: the pattern needs as many columns as the matrix");
+                uc = U.colptr;
+                ui = U.rowind;
+            }
+            if (ui.empty()) ui.push_back(0);
+            check(mspai_set_upper_pattern(ctx, uc.data(), ui.data()), ctx, "mspai_set_upper_pattern");
+        }
         if (o.binning == 2 && bs == 1) check(mspai_autotune(ctx, &o, 1, nullptr), ctx, "mspai_autotune");
         mspai_stats st;
         check(mspai_build(ctx, &o, &st), ctx, "mspai_build");

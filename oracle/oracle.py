@@ -21,7 +21,8 @@ ST_SINGULAR, ST_UNDERDET, ST_NAN_RHO, ST_K_NOT_IN_I = 1, 2, 4, 16
 
 class _Opts(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("maxnew", C.c_int), ("max_impr_steps", C.c_int),
-                ("use_mean", C.c_int), ("qr", C.c_int), ("cache", C.c_int), ("threads", C.c_int)]
+                ("use_mean", C.c_int), ("qr", C.c_int), ("cache", C.c_int), ("threads", C.c_int),
+                ("up_ptr", C.c_void_p), ("up_ind", C.c_void_p)]
 
 
 class _Csc(C.Structure):
@@ -68,11 +69,13 @@ def capacity(prob) -> int:
 
 
 def spai(prob, cols=None, threads: int = 1, qr: int = 0, cache: int = 0, epsilon=None,
-         maxnew=None, max_impr_steps=None, use_mean=None):
+         maxnew=None, max_impr_steps=None, use_mean=None, up=None):
     """Run the oracle on `prob` (a mspai_amd.gen.Problem); returns a dict with
     len, idx, val (padded to `cap`), res_norm, nsolve, status, work, cap, and
     near (per column: near-margin decisions at the mean, at the maxnew cut,
-    at epsilon; fresh-QR path only)."""
+    at epsilon; fresh-QR path only).  `up`: an upper pattern (a sparse
+    matrix with A's column count; its stored positions), which restricts
+    every column's candidates to its column (-up)."""
     L = lib()
     keep = []
     A = _csc_struct(prob.A, keep)
@@ -84,6 +87,16 @@ def spai(prob, cols=None, threads: int = 1, qr: int = 0, cache: int = 0, epsilon
               prob.maxnew if maxnew is None else maxnew,
               prob.max_impr_steps if max_impr_steps is None else max_impr_steps,
               prob.use_mean if use_mean is None else use_mean, qr, cache, threads)
+    if up is not None:
+        import scipy.sparse as sp
+        U = sp.csc_matrix(up)
+        U.sort_indices()
+        if U.shape[1] != prob.A.shape[1]:
+            raise ValueError("upper pattern needs A's column count")
+        up_ptr = np.ascontiguousarray(U.indptr, dtype=np.int32)
+        up_ind = np.ascontiguousarray(U.indices, dtype=np.int32)
+        keep += [up_ptr, up_ind]
+        o.up_ptr, o.up_ind = up_ptr.ctypes.data, up_ind.ctypes.data
     ns = o.max_impr_steps
     plen = np.diff(prob.P.indptr)
     cap = int(plen.max()) + o.maxnew * max(0, ns)

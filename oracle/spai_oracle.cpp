@@ -286,7 +286,8 @@ int make_rhs(const Problem& P, Work& w, int col)
 
 /* Union_Nl_Sets (src/Spai_Sub.imp:124-200) over L = I u {col}
  * (src/Spai_Sub.imp:1044-1059), then J~ = U \ J (std::set_difference,
- * src/Spai_Sub.imp:1066-1069). */
+ * src/Spai_Sub.imp:1066-1069), then with an upper pattern J~ n UP(:,col)
+ * (Index_Set::Set_Intersection, src/Spai_Sub.imp:1072-1094). */
 void candidates(const Problem& P, Work& w, int col)
 {
     w.L.assign(w.I.begin(), w.I.end());
@@ -313,6 +314,12 @@ void candidates(const Problem& P, Work& w, int col)
     w.Jt.resize(w.U.size());
     auto last = std::set_difference(w.U.begin(), w.U.end(), w.J.begin(), w.J.end(), w.Jt.begin());
     w.Jt.resize(last - w.Jt.begin());
+    if (P.o->up_ptr) {
+        const int* u0 = P.o->up_ind + P.o->up_ptr[col];
+        const int* u1 = P.o->up_ind + P.o->up_ptr[col + 1];
+        auto kept = std::set_intersection(w.Jt.begin(), w.Jt.end(), u0, u1, w.Jt.begin());
+        w.Jt.resize(kept - w.Jt.begin());
+    }
 }
 
 /* Compute_Rhos (src/Spai_Sub.imp:1231-1348) with Compute_Numerator

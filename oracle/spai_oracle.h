@@ -50,6 +50,8 @@ typedef struct {
     int qr;               /* -qr: 0 = plain, 2 = dense QR update */
     int cache;            /* -cs: LRU cache entries, 0 = off */
     int threads;          /* host threads, contiguous split of the column list */
+    const int* up_ptr;    /* -up: upper pattern (CSC pattern, sorted), NULL = none */
+    const int* up_ind;
 } orc_opts;
 
 typedef struct {

@@ -428,6 +428,59 @@ def test_reuse_pattern_static_resolve(oracle, gpu):
     assert_bitwise(oracle, prob2, ref, M2, info)
 
 
+def _upper_pattern(n, per_col, seed):
+    """Nonsymmetric random upper pattern: the diagonal and `per_col` random
+    indices per column, drawn near the column (where the candidates of a
+    banded or local matrix lie) and anywhere."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rows, cols = [np.arange(n)], [np.arange(n)]
+    for spread in (64, n):
+        k = np.repeat(np.arange(n), per_col // 2)
+        off = rng.integers(-spread, spread + 1, size=k.size)
+        rows.append(np.clip(k + off, 0, n - 1))
+        cols.append(k)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    U = sp.csc_matrix((np.ones(r.size), (r, c)), shape=(n, n))
+    U.sum_duplicates()
+    U.sort_indices()
+    return U
+
+
+@pytest.mark.parametrize("name", ["c2_convdiff_32", "c3_random_8192", "cs_fs_183_1"])
+@pytest.mark.parametrize("binning,qr", [(0, 0), (1, 0), (1, 2)])
+def test_upper_pattern_bitwise(oracle, gpu, name, binning, qr):
+    """-up: candidates restricted to UP(:, k) (src/Spai_Sub.imp:1072-1094);
+    M, ||r||, solves, status and work counters equal the oracle's with the
+    same upper pattern, on the fresh-QR and the QR-update path.  Builds with
+    an upper pattern run unbinned (the generic kernel) whatever the binning
+    option, and a mixed request builds in FP64; removing the upper pattern
+    gives the unrestricted build again."""
+    prob = CASES[name]()
+    UP = _upper_pattern(prob.n, 24, 5)
+    ref = oracle.spai(prob, threads=THREADS, qr=qr, up=UP)
+    b = gpu.Builder(0)
+    try:
+        b.upload(prob.A, prob.row_pattern())
+        b.set_pattern(prob.P)
+        b.set_upper_pattern(UP)
+        print(f"[upper pattern {name} binning={binning} qr={qr}]", flush=True)
+        b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps, binning=binning,
+                qr=qr, counters=1)
+        M, info = b.download(), b.column_info()
+        assert_bitwise(oracle, prob, ref, M, info)
+        assert not same_pattern(M, oracle.to_csc(oracle.spai(prob, threads=THREADS, qr=qr), prob.n))
+        if qr == 0 and binning == 1:
+            b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps, binning=1,
+                    precision=1)
+            assert same_csc_bitwise(b.download(), M)
+        b.set_upper_pattern(None)
+        b.build(epsilon=prob.epsilon, maxnew=prob.maxnew, max_impr_steps=prob.max_impr_steps, binning=binning,
+                qr=qr)
+        assert same_csc_bitwise(b.download(), oracle.to_csc(oracle.spai(prob, threads=THREADS, qr=qr), prob.n))
+    finally:
+        b.close()
+
+
 @pytest.mark.parametrize("name", ["cs_fs_183_1", "cs_bcsstk01", "cs_ibm32a", "c3_random_8192"])
 def test_launch_stats_and_near_margins(oracle, gpu, name):
     """Per-launch totals of a build with counters: the launches together

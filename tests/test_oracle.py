@@ -131,14 +131,39 @@ def _rho_bruteforce(A, r_ext, j):
 def test_one_augmentation_step_bruteforce(oracle, make):
     """I-set, candidates, rho, the mean threshold and the top-mn selection of
     one augmentation step, recomputed independently with dense numpy."""
+    _one_step_bruteforce(oracle, make())
+
+
+def _random_upper_pattern(n, density, seed):
+    """A nonsymmetric random pattern with the diagonal: column k allows the
+    rows it stores (the candidates j of column k)."""
+    U = sp.random(n, n, density=density, random_state=seed, format="csc")
+    U = (U + sp.identity(n, format="csc")).tocsc()
+    U.data[:] = 1.0
+    U.sort_indices()
+    return U
+
+
+@pytest.mark.parametrize("make", [lambda: gen.config2(24), lambda: gen.config3(n=3000, bw=300)])
+def test_one_augmentation_step_upper_pattern_bruteforce(oracle, make):
+    """With an upper pattern (-up) the candidates of column k are those of
+    the plain step that UP(:, k) holds (Set_Intersection,
+    src/Spai_Sub.imp:1072-1094); rho, the mean over them and the selection
+    recomputed with dense numpy.  UP is nonsymmetric, so intersecting with
+    a row of UP instead of its column fails."""
     p = make()
+    UP = _random_upper_pattern(p.n, 0.6 if p.n < 1000 else 0.02, 7)
+    _one_step_bruteforce(oracle, p, UP)
+
+
+def _one_step_bruteforce(oracle, p, UP=None):
     A = sp.csc_matrix(p.A)
     # structural pattern (stored entries, explicit zeros included), as the
     # reference's index sets are built from stored indices
     Ad = sp.csc_matrix((np.ones(A.nnz), A.indices, A.indptr), shape=A.shape).toarray()
     n = p.n
     mn = p.maxnew
-    one = oracle.spai(p, max_impr_steps=1, epsilon=0.0)
+    one = oracle.spai(p, max_impr_steps=1, epsilon=0.0, up=UP)
     rng = np.random.Generator(np.random.PCG64(5))
     cols = rng.choice(n, size=min(n, 60), replace=False)
     checked = 0
@@ -152,6 +177,8 @@ def test_one_augmentation_step_bruteforce(oracle, make):
         r_ext[I] = r
         L = np.union1d(I, [k])
         cand = np.setdiff1d(np.nonzero(np.any(Ad[L, :] != 0, axis=0))[0], J0)
+        if UP is not None:
+            cand = np.intersect1d(cand, UP.indices[UP.indptr[k]:UP.indptr[k + 1]])
         J1 = one["idx"][k, :one["len"][k]]
         added = np.setdiff1d(J1, J0)
         if cand.size == 0:
@@ -171,6 +198,28 @@ def test_one_augmentation_step_bruteforce(oracle, make):
         assert np.array_equal(np.sort(added), np.sort(cand[order[:s]])), k
         checked += 1
     assert checked >= 10
+
+
+def test_upper_pattern_limits(oracle):
+    """A full upper pattern changes nothing, bit for bit; an empty one leaves
+    every column at its static solve on J0; a random one keeps every column
+    of M inside J0 u UP(:, k)."""
+    p = gen.config2(12)
+    n = p.n
+    ref = oracle.to_csc(oracle.spai(p), n)
+    full = sp.csc_matrix(np.ones((n, n)))
+    assert same_csc_bitwise(oracle.to_csc(oracle.spai(p, up=full), n), ref)
+    empty = sp.csc_matrix((n, n))
+    static = oracle.to_csc(oracle.spai(p, max_impr_steps=0), n)
+    assert same_csc_bitwise(oracle.to_csc(oracle.spai(p, up=empty), n), static)
+    UP = _random_upper_pattern(n, 0.3, 3)
+    out = oracle.spai(p, up=UP)
+    M = oracle.to_csc(out, n)
+    assert not same_csc_bitwise(M, ref)
+    P = sp.csc_matrix(p.P)
+    for k in range(n):
+        allowed = np.union1d(P.indices[P.indptr[k]:P.indptr[k + 1]], UP.indices[UP.indptr[k]:UP.indptr[k + 1]])
+        assert np.all(np.isin(M.indices[M.indptr[k]:M.indptr[k + 1]], allowed)), k
 
 
 @pytest.mark.parametrize("make", [lambda: gen.config2(32), lambda: gen.config3(n=4096, bw=400),

@@ -45,6 +45,35 @@ def test_cli_writes_reference_layout(oracle, tmp_path, name):
     assert out.read_text() == format_mtx(oracle.to_csc(ref, n))
 
 
+@pytest.mark.parametrize("up", ["2", "file"])
+def test_cli_upper_pattern(oracle, tmp_path, up):
+    """-up 2 (the pattern of A) and -up <file> (a MatrixMarket pattern)
+    through the command line, against the oracle with the same upper
+    pattern."""
+    import numpy as np
+    import scipy.sparse as sp
+    A = gen.config2(32).A.tocsc()
+    n = A.shape[1]
+    src, out = tmp_path / "a.mtx", tmp_path / "precond.mtx"
+    write_input_mtx(src, A, symmetric=False)
+    if up == "file":
+        UP = (sp.random(n, n, density=0.02, random_state=4, format="csc") + sp.identity(n, format="csc")).tocsc()
+        UP.data[:] = 1.0
+        write_input_mtx(tmp_path / "up.mtx", UP, symmetric=False)
+        arg = str(tmp_path / "up.mtx")
+    else:
+        UP = gen.pattern_of(A)
+        arg = "2"
+    args = [CLI, str(src), "-pp", "1", "-up", arg, "-ep", "0.4", "-mn", "5", "-ns", "5", "-out", str(out)]
+    print(f"[cli upper pattern {up}]", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    prob = gen.Problem("up", A, gen.diag_pattern(n), None, epsilon=0.4, maxnew=5, max_impr_steps=5)
+    ref = oracle.spai(prob, threads=THREADS, up=UP)
+    assert out.read_text() == format_mtx(oracle.to_csc(ref, n))
+    assert not np.array_equal(oracle.to_csc(ref, n).indptr, oracle.to_csc(oracle.spai(prob, threads=THREADS), n).indptr)
+
+
 BLOCK_CASES = {
     # name: (matrix, -bs, -pp, options)
     "block3_adaptive": (lambda: gen.block_convdiff2d(6, 3, seed=21), 3, 1, dict(ep=0.3, mn=3, ns=3)),

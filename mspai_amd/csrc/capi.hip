@@ -101,21 +101,41 @@ __global__ void work_sum_kernel(const double* work, int n, double* out)
 /* M's CSC from the per-column outputs: colptr[1..ncols] holds the inclusive
  * scan of the lengths; this kernel also sets colptr[0] = 0 and *nnz, and
  * counts the columns the FP64 kernels built in a mixed build (*nfp64). */
+/* Output slots -> CSC: kScatterLanes lanes per column (most columns hold
+ * fewer entries), colptr from the scan of the lengths. */
+constexpr int kScatterLanes = 16;
+
 __global__ void scatter_columns_kernel(int ncols, int cap, const int* m_len, const int* m_idx,
-                                       const double* m_val, int* colptr, int* rowind, double* val, int* nnz,
-                                       const int* status, int* nfp64)
+                                       const double* m_val, int* colptr, int* rowind, double* val, int* nnz)
 {
-    int col = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    int lane = threadIdx.x & 63;
+    const int col = blockIdx.x * (blockDim.x / kScatterLanes) + (int)(threadIdx.x / kScatterLanes);
+    const int sub = threadIdx.x % kScatterLanes;
     if (col >= ncols) return;
-    int len = m_len[col], o = col == 0 ? 0 : colptr[col];
-    if (lane == 0 && col == 0) colptr[0] = 0;
-    if (lane == 0 && col == ncols - 1) *nnz = o + len;
-    if (lane == 0 && nfp64 && (status[col] & COL_FALLBACK)) atomicAdd(nfp64, 1);
-    size_t src = (size_t)col * cap;
-    for (int i = lane; i < len; i += 64) {
+    const int len = m_len[col], o = col == 0 ? 0 : colptr[col];
+    if (sub == 0 && col == 0) colptr[0] = 0;
+    if (sub == 0 && col == ncols - 1) *nnz = o + len;
+    const size_t src = (size_t)col * cap;
+    for (int i = sub; i < len; i += kScatterLanes) {
         rowind[o + i] = m_idx[src + i];
         val[o + i] = m_val[src + i];
+    }
+}
+
+/* *out += the columns whose status has `bit`: a grid-stride count, one
+ * atomic per block (a per-column atomic on one address serializes). */
+__global__ __launch_bounds__(256) void count_status_kernel(int n, const int* status, int bit, int* out)
+{
+    __shared__ int part[256 / 64];
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        c += (status[i] & bit) != 0;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 256 / 64; ++w) t += part[w];
+        if (t) atomicAdd(out, t);
     }
 }
 
@@ -1419,9 +1439,13 @@ extern "C" int mspai_build(mspai_ctx* ctx, const mspai_opts* opts, mspai_stats* 
     size_t cub_bytes = ctx->cub_bytes;
     HIPCHK(hipcub::DeviceScan::InclusiveSum(ctx->d_cub, cub_bytes, ctx->d_m_len, ctx->d_Mptr + 1, n,
                                              ctx->stream));
-    hipLaunchKernelGGL(scatter_columns_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, cap,
-                       ctx->d_m_len, ctx->d_m_idx, ctx->d_m_val, ctx->d_Mptr, ctx->d_Mind, ctx->d_Mval,
-                       ctx->d_counter + 63, ctx->d_status, mixed ? ctx->d_counter + 62 : nullptr);
+    constexpr int kScatterCols = 256 / kScatterLanes;
+    hipLaunchKernelGGL(scatter_columns_kernel, dim3((n + kScatterCols - 1) / kScatterCols), dim3(256), 0,
+                       ctx->stream, n, cap, ctx->d_m_len, ctx->d_m_idx, ctx->d_m_val, ctx->d_Mptr, ctx->d_Mind,
+                       ctx->d_Mval, ctx->d_counter + 63);
+    if (mixed)
+        hipLaunchKernelGGL(count_status_kernel, dim3(std::min(256, (n + 255) / 256)), dim3(256), 0, ctx->stream, n,
+                           ctx->d_status, COL_FALLBACK, ctx->d_counter + 62);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     if ((rc = fetch_counters(ctx))) return rc;

@@ -2964,7 +2964,9 @@ __device__ __forceinline__ int distinct_rows_upto(const BuildArgs& a, int p0, in
  * capacities hold its start pattern (|I| is counted exactly when the raw list
  * exceeds the bin's |I| and |J0| <= 8; a column that still does not fit,
  * e.g. after augmentation, moves on by the overflow path). */
-__global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, BinEdges edges, int* queues,
+constexpr int kClassifyBlock = 1024;
+
+__global__ __launch_bounds__(kClassifyBlock) void classify_kernel(BuildArgs a, int nbins, BinEdges edges, int* queues,
                                                        int* counts, int* gqueue, int* gcount, int n, int* done,
                                                        int* routed)
 {
@@ -2994,8 +2996,10 @@ __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, B
             break;
         }
     }
-    /* one append per queue and wave, the atomics of all queues issued
-     * together: lane b adds the wave's count of queue b */
+    /* one append per queue and block: lane b of each wave reserves the
+     * wave's share of queue b in LDS, then thread b adds the block's count
+     * of queue b to the queue (a per-wave atomic on the few queue counters
+     * serializes thousands of times) */
     const int lane = lane_id();
     const bool live = col < n;
     unsigned long long mine = 0ull, lanes_b = 0ull;
@@ -3004,12 +3008,20 @@ __global__ __launch_bounds__(256) void classify_kernel(BuildArgs a, int nbins, B
         if (bin == b) mine = mb;
         if (lane == b) lanes_b = mb;
     }
-    int base = 0;
-    if (lane <= nbins && lanes_b != 0ull) base = atomicAdd(lane < nbins ? counts + lane : gcount, __popcll(lanes_b));
-    base = __shfl(base, bin);
+    constexpr int kQueues = sizeof(BinEdges::capJ) / sizeof(int) + 1;  // the bins and the generic queue
+    __shared__ int bcount[kQueues], bbase[kQueues];
+    if ((int)threadIdx.x <= nbins) bcount[threadIdx.x] = 0;
+    __syncthreads();
+    int woff = 0;
+    if (lane <= nbins && lanes_b != 0ull) woff = atomicAdd(&bcount[lane], __popcll(lanes_b));
+    __syncthreads();
+    if ((int)threadIdx.x <= nbins && bcount[threadIdx.x] > 0)
+        bbase[threadIdx.x] = atomicAdd((int)threadIdx.x < nbins ? counts + threadIdx.x : gcount, bcount[threadIdx.x]);
+    __syncthreads();
+    const int off = __shfl(woff, bin);
     if (live) {
         int* out = bin < nbins ? queues + (size_t)bin * n : gqueue;
-        out[base + __popcll(mine & ((1ull << lane) - 1ull))] = col;
+        out[bbase[bin] + off + __popcll(mine & ((1ull << lane) - 1ull))] = col;
     }
     /* the last block to finish records the routed queue lengths */
     __shared__ int last;
@@ -3118,8 +3130,8 @@ void launch_binned(const BuildArgs& a, const ScratchLayout& L, int group, int mo
 void launch_classify(const BuildArgs& a, int nbins, const BinEdges& edges, int* queues, int* counts,
                      int* gqueue, int* gcount, int n, int* done, int* routed, hipStream_t stream)
 {
-    hipLaunchKernelGGL(classify_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, nbins, edges, queues,
-                       counts, gqueue, gcount, n, done, routed);
+    hipLaunchKernelGGL(classify_kernel, dim3((n + kClassifyBlock - 1) / kClassifyBlock), dim3(kClassifyBlock), 0,
+                       stream, a, nbins, edges, queues, counts, gqueue, gcount, n, done, routed);
 }
 
 void launch_copy_ints(const int* src, int* dst, int count, hipStream_t stream)
